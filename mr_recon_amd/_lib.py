@@ -1,0 +1,109 @@
+"""ctypes binding of the HIP C-ABI library ``libmr_recon_amd.so``.
+
+Every compute entry point takes raw device pointers plus the HIP stream that
+PyTorch is currently using, so launches order correctly with torch work and can
+be captured in a HIP graph. There is no CPU fallback: calling an op without the
+library, or with tensors that are not on a ROCm device, raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "libmr_recon_amd.so"
+
+_lock = threading.Lock()
+_lib = None
+
+_vp = C.c_void_p
+_i = C.c_int
+_ll = C.c_longlong
+_f = C.c_float
+_ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+_llp = C.POINTER(C.c_longlong)
+
+# name -> (restype, argtypes); mirrors include/mr_recon_amd.h
+SIGNATURES = {
+    "mra_version": (_i, []),
+    "mra_last_error": (C.c_char_p, []),
+    "mra_geom_size": (_i, []),
+    "mra_default_tile": (_i, [_i]),
+    "mra_samples_per_item": (_i, []),
+    "mra_geom_fill": (_i, [_vp, _ll, _ll, _ll, _ll, _ll, _i, _i, _i, _ip, _i, _ip, _f, _f, _f]),
+    "mra_scan_workspace": (_ll, [_ll]),
+    "mra_scan_excl_i32": (_i, [_vp, _ll, _vp, _vp, _vp]),
+    "mra_bin_sizes": (_i, [_ll, _i, _i, _ip, _llp, _llp]),
+    "mra_bin_build": (_i, [_i, _vp, _ll, _ip, _fp, _fp, _i, _f, _i, _ip, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mra_bin_items": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp]),
+    "mra_interp": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "mra_grid": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "mra_interp_grad": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "mra_pad_expand": (_i, [_i, _ip, _ip, _ip, _vp, _ll, _ll, _vp, _ll, _ll, _vp, _f, _vp, _vp]),
+    "mra_crop_reduce": (_i, [_i, _ip, _ip, _ip, _vp, _ll, _vp, _ll, _ll, _i, _vp, _f, _i, _i, _vp, _ll, _vp]),
+    "mra_cmul_bcast": (_i, [_vp, _vp, _ll, _ll, _vp]),
+    "mra_cg_partials": (_i, []),
+    "mra_cdot_re": (_i, [_vp, _vp, _ll, _f, _vp, _vp, _i, _vp]),
+    "mra_cg_init": (_i, [_vp, _vp, _vp, _vp, _vp, _ll, _f, _vp, _vp, _vp]),
+    "mra_cg_step": (_i, [_vp, _vp, _vp, _vp, _ll, _f, _vp, _vp, _vp]),
+}
+
+
+def lib():
+    """Load (once) and return the ctypes library handle."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(
+                    f"HIP extension not built ({LIB_PATH} missing); run `python -m mr_recon_amd.csrc.build`"
+                )
+            h = C.CDLL(str(LIB_PATH))
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(h, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = h
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != 0:
+        raise RuntimeError(f"mr_recon_amd HIP call failed: {lib().mra_last_error().decode()}")
+
+
+def require_device(*tensors: torch.Tensor) -> None:
+    """The HIP path only runs on ROCm device tensors; there is no CPU fallback."""
+    for t in tensors:
+        if t is None:
+            continue
+        if not t.is_cuda:
+            raise RuntimeError(
+                "mr_recon_amd kernels run on the GPU only; got a tensor on "
+                f"{t.device}. Move inputs to a ROCm device (e.g. 'cuda:0')."
+            )
+
+
+def ptr(t: torch.Tensor | None):
+    return None if t is None else _vp(t.data_ptr())
+
+
+def stream_ptr(device: torch.device | None = None):
+    return _vp(torch.cuda.current_stream(device).cuda_stream)
+
+
+def ints(vals) -> C.Array:
+    vals = [int(v) for v in vals]
+    return (C.c_int * len(vals))(*vals)
+
+
+def floats(vals) -> C.Array:
+    vals = [float(v) for v in vals]
+    return (C.c_float * len(vals))(*vals)

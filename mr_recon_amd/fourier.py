@@ -1,0 +1,373 @@
+"""NUFFT call surface on the HIP kernels.
+
+Mirrors the reference ``mr_recon.fourier`` for the hot path:
+  ``fft`` / ``ifft``            fourier.py:37-61   (centred FFTs, plain torch)
+  ``calc_toep_kernel_helper``   fourier.py:63-105
+  ``NUFFT``                     fourier.py:107-352 (base class, ``normal_toeplitz``)
+  ``hip_nufft``                 fourier.py:354-582 (``triton_nufft``; alias kept)
+
+Transform definition (the reference's): forward
+    ksp = interp_KB( FFT( pad_G( apod * img / sqrt(prod N) ) ) ) / w^d
+with the pad centred (utils.resize) and the FFT centred (ifftshift / fftshift),
+and the adjoint its exact conjugate transpose. ``forward`` and ``adjoint`` run
+fused: one pad/apodize kernel writes the oversampled grid straight into the
+unshifted FFT layout, rocFFT runs in place, and the KB kernels address that
+layout through an index offset, so no shift, pad or crop copy is made. The
+split methods (``forward_FT_only`` etc.) keep the reference's centred layout of
+the intermediate grid so they compose with code written against it.
+"""
+
+from __future__ import annotations
+
+import math
+from collections import OrderedDict
+from math import ceil
+from typing import Optional
+
+import numpy as np
+import torch
+import torch.fft as fft_torch
+import torch.nn as nn
+
+from . import _lib, ops
+from .dtypes import complex_dtype, real_dtype
+from .pad import PadLast
+from .plan import SamplePlan
+from .utils import batch_iterator, resize
+
+
+def fft(x, dim=None, oshape=None, norm="ortho"):
+    """Centred FFT: fftshift(fftn(ifftshift(x))) (reference fourier.py:37-47)."""
+    if oshape is not None:
+        x = resize(x, oshape)
+    x = fft_torch.ifftshift(x, dim=dim)
+    x = fft_torch.fftn(x, dim=dim, norm=norm)
+    return fft_torch.fftshift(x, dim=dim)
+
+
+def ifft(x, dim=None, oshape=None, norm="ortho"):
+    """Centred inverse FFT (reference fourier.py:50-61)."""
+    if oshape is not None:
+        x = resize(x, oshape)
+    x = fft_torch.ifftshift(x, dim=dim)
+    x = fft_torch.ifftn(x, dim=dim, norm=norm)
+    return fft_torch.fftshift(x, dim=dim)
+
+
+def calc_toep_kernel_helper(nufft_adj_os: callable, trj: torch.Tensor, weights: Optional[torch.Tensor] = None):
+    """PSF kernels: centred FFT of the adjoint NUFFT of ``weights`` (reference fourier.py:63-105)."""
+    dev = trj.device
+    if weights is None:
+        weights = torch.ones(trj.shape[:-1], dtype=real_dtype, device=dev)
+    else:
+        assert weights.device == dev
+    trj_batch = trj.shape[1:-1]
+    if trj.shape[0] == 1 and weights.shape[0] != 1:
+        trj = trj.expand((weights.shape[0], *trj.shape[1:]))
+    N = trj.shape[0]
+    d = trj.shape[-1]
+    ksp = torch.ones((N, 1, *trj_batch), device=dev, dtype=complex_dtype)
+    ksp_weighted = ksp * weights[:, None, ...]
+    img = nufft_adj_os(ksp_weighted, trj)[:, 0, ...]
+    return fft(img, dim=tuple(range(-d, 0)))
+
+
+def _fftn_(x: torch.Tensor, d: int) -> torch.Tensor:
+    """Unnormalised forward FFT over the last d dims, in place."""
+    return fft_torch.fftn(x, dim=tuple(range(-d, 0)), out=x)
+
+
+def _ifftn_(x: torch.Tensor, d: int) -> torch.Tensor:
+    """Unnormalised inverse FFT (no 1/n) over the last d dims, in place."""
+    return fft_torch.ifftn(x, dim=tuple(range(-d, 0)), norm="forward", out=x)
+
+
+class NUFFT(nn.Module):
+    """Forward NUFFT e^{-j 2 pi k . r}, k_i in [-N_i/2, N_i/2], r_i in [-1/2, 1/2] (fourier.py:107-352)."""
+
+    def __init__(self, im_size: tuple):
+        super().__init__()
+        self.im_size = tuple(im_size)
+
+    def rescale_trajectory(self, trj: torch.Tensor) -> torch.Tensor:
+        return trj
+
+    def forward_FT_only(self, img):
+        raise NotImplementedError
+
+    def adjoint_iFT_only(self, ksp_os):
+        raise NotImplementedError
+
+    def adjoint_grid_only(self, ksp, trj):
+        raise NotImplementedError
+
+    def forward_interp_only(self, ksp_os, trj):
+        raise NotImplementedError
+
+    def forward(self, img: torch.Tensor, trj: torch.Tensor) -> torch.Tensor:
+        return self.forward_interp_only(self.forward_FT_only(img), trj)
+
+    def adjoint(self, ksp: torch.Tensor, trj: torch.Tensor) -> torch.Tensor:
+        return self.adjoint_iFT_only(self.adjoint_grid_only(ksp, trj))
+
+    def calc_teoplitz_kernels(self, trj, weights=None, os_factor=2.0):
+        raise NotImplementedError
+
+    def normal_toeplitz(self, img: torch.Tensor, toeplitz_kernels: torch.Tensor) -> torch.Tensor:
+        """A^H A img through the PSF kernels (reference fourier.py:302-352).
+
+        Same composition as the reference (PadLast pad, centred ortho FFT,
+        multiply, centred ortho iFFT, crop), run as pad kernel -> in-place FFT ->
+        multiply kernel -> in-place iFFT -> crop kernel in the unshifted layout.
+        """
+        _lib.require_device(img, toeplitz_kernels)
+        N = img.shape[0]
+        d = len(self.im_size)
+        im_size_os = tuple(toeplitz_kernels.shape[1:])
+        img_batch = img.shape[1:-d]
+        B = int(math.prod(img_batch))
+        padder = PadLast(im_size_os, self.im_size)
+        cen = padder.center
+        out = torch.empty((N, B, *self.im_size), dtype=complex_dtype, device=img.device)
+        for n in range(N):
+            kern = toeplitz_unshifted(toeplitz_kernels[n])
+            x = img[n].reshape(B, *self.im_size).to(complex_dtype)
+            g = ops.pad_expand(x, self.im_size, im_size_os, center=cen)
+            _fftn_(g, d)
+            ops.cmul_bcast(g, kern)
+            _ifftn_(g, d)
+            ops.crop_reduce(g, self.im_size, im_size_os, B, center=cen, out=out[n])
+        return out.reshape(N, *img_batch, *self.im_size)
+
+
+def toeplitz_unshifted(kern: torch.Tensor) -> torch.Tensor:
+    """Centred ortho-normalised PSF kernel -> multiplier for unnormalised FFTs in the unshifted layout."""
+    d = kern.ndim
+    P = int(math.prod(kern.shape))
+    return (fft_torch.ifftshift(kern, dim=tuple(range(-d, 0))) / P).to(complex_dtype).contiguous()
+
+
+def kb_beta(width: float, oversamp: float) -> float:
+    """Beatty et al. shape parameter, as the reference (fourier.py:372)."""
+    return float(np.pi * (((width / oversamp) * (oversamp - 0.5)) ** 2 - 0.8) ** 0.5)
+
+
+class _PlanCache:
+    """LRU of sample plans keyed by the trajectory tensor's memory and version.
+
+    Each entry holds a reference to the trajectory tensor, so its memory cannot
+    be recycled for another trajectory while the plan is cached; in-place edits
+    bump the version counter and miss the cache.
+    """
+
+    def __init__(self, size: int = 4):
+        self.size = size
+        self.d: OrderedDict = OrderedDict()
+
+    def get(self, trj: torch.Tensor, extra, build):
+        key = (trj.data_ptr(), trj._version, tuple(trj.shape), tuple(trj.stride()), trj.dtype, str(trj.device),
+               extra)
+        hit = self.d.get(key)
+        if hit is not None:
+            self.d.move_to_end(key)
+            return hit[0]
+        plan = build()
+        self.d[key] = (plan, trj)
+        while len(self.d) > self.size:
+            self.d.popitem(last=False)
+        return plan
+
+
+class hip_nufft(NUFFT):
+    """Kaiser-Bessel NUFFT on hand-written HIP kernels (drop-in for ``triton_nufft``, fourier.py:354-582)."""
+
+    def __init__(self, im_size: tuple, oversamp: Optional[float] = 1.25, width: Optional[int] = 6,
+                 beta: Optional[float] = None, apodize: Optional[bool] = True,
+                 samples_per_item: Optional[int] = None):
+        super().__init__(im_size)
+        self.oversamp = oversamp
+        self.width = width
+        self.apodize = apodize
+        self.beta = kb_beta(width, oversamp) if beta is None else beta
+        self.samples_per_item = samples_per_item
+        self._plans = _PlanCache()
+        self._apod_cache: dict = {}
+
+    # -- geometry ------------------------------------------------------
+    def os_shape(self, im_size=None):
+        im_size = self.im_size if im_size is None else im_size
+        return tuple(ceil(self.oversamp * n) for n in im_size)
+
+    def _scale_shift(self, im_size):
+        """Per-dim (fl32(ceil(sN)/N), ceil(sN)//2) of ``_scale_trj`` (fourier.py:420-430)."""
+        sc = [float(np.float32(ceil(self.oversamp * n) / n)) for n in im_size]
+        sh = [float(ceil(self.oversamp * n) // 2) for n in im_size]
+        return sc, sh
+
+    def _apodization_func(self, x: torch.Tensor) -> torch.Tensor:
+        """sqrt(beta^2 - (pi w x)^2) / sinh(sqrt(...))   (fourier.py:376-396)."""
+        apod = (self.beta ** 2 - (np.pi * self.width * x) ** 2) ** 0.5
+        return apod / torch.sinh(apod)
+
+    def apod_table(self, device, im_size=None) -> Optional[torch.Tensor]:
+        """Concatenated per-dim apodization (pixel n: x = (n - N//2) / ceil(s N))."""
+        if not self.apodize:
+            return None
+        im_size = tuple(self.im_size if im_size is None else im_size)
+        key = (str(device), im_size)
+        t = self._apod_cache.get(key)
+        if t is None:
+            parts = []
+            for n in im_size:
+                crds = torch.arange(n, dtype=torch.float64) - n // 2
+                parts.append(self._apodization_func(crds / ceil(self.oversamp * n)))
+            t = torch.cat(parts).to(torch.float32).to(device)
+            self._apod_cache[key] = t
+        return t
+
+    def apodize_img(self, img: torch.Tensor) -> torch.Tensor:
+        """In-place apodization of the trailing image dims (fourier.py:398-418)."""
+        ndim = len(self.im_size)
+        for i in range(-ndim, 0):
+            crds = torch.arange(img.shape[i], device=img.device) - img.shape[i] // 2
+            tup = (slice(None),) + (None,) * (-i - 1)
+            img *= self._apodization_func(crds / ceil(self.oversamp * img.shape[i]))[tup]
+        return img
+
+    def plan(self, trj_n: torch.Tensor, grid_size=None, centred: bool = False, im_size=None) -> SamplePlan:
+        """Sample plan of one trajectory (trj_n: (*trj_batch, d), reference units).
+
+        ``centred`` plans address a centred grid (the public split methods);
+        otherwise the unshifted FFT output (the fused paths).
+        """
+        im_size = tuple(self.im_size if im_size is None else im_size)
+        G = tuple(self.os_shape(im_size) if grid_size is None else grid_size)
+        wrap = (0,) * len(G) if centred else tuple(-(g // 2) for g in G)
+        sc, sh = self._scale_shift(im_size)
+        t = trj_n.detach()
+        if t.dtype != torch.float32:
+            t = t.to(torch.float32)
+        return self._plans.get(t, (G, wrap, im_size, self.width, self.beta, self.samples_per_item),
+                               lambda: SamplePlan(t, G, self.width, self.beta, scale=sc, shift=sh, wrap=wrap,
+                                                  samples_per_item=self.samples_per_item))
+
+    # -- split methods (reference layout) -------------------------------
+    def forward_FT_only(self, img: torch.Tensor) -> torch.Tensor:
+        """apodize -> / sqrt(prod N) -> centred zero-pad -> centred FFT (norm=None)."""
+        _lib.require_device(img)
+        d = len(self.im_size)
+        im = tuple(img.shape[-d:])
+        G = self.os_shape(im)
+        lead = img.shape[:-d]
+        x = img.reshape(-1, *im).to(complex_dtype)
+        g = ops.pad_expand(x, im, G, apod=self.apod_table(img.device, im), scale=1.0 / math.sqrt(math.prod(im)))
+        _fftn_(g, d)
+        g = fft_torch.fftshift(g, dim=tuple(range(-d, 0)))
+        return g.reshape(*lead, *G)
+
+    def forward_interp_only(self, ksp_os: torch.Tensor, trj: torch.Tensor) -> torch.Tensor:
+        """KB interpolation of a centred grid at ``trj``, / w^d."""
+        _lib.require_device(ksp_os, trj)
+        d = len(self.im_size)
+        G = tuple(ksp_os.shape[-d:])
+        N = trj.shape[0]
+        lead = ksp_os.shape[1:-d]
+        B = int(math.prod(lead))
+        trj_batch = trj.shape[1:-1]
+        M = int(math.prod(trj_batch))
+        out = torch.empty((N, B, M), dtype=complex_dtype, device=ksp_os.device)
+        src = ksp_os.to(complex_dtype).reshape(N, B, *G)
+        for i in range(N):
+            plan = self.plan(trj[i], grid_size=G, centred=True)
+            ops.interp(src[i], plan, out_scale=1.0 / self.width ** d, out=out[i])
+        return out.reshape(N, *lead, *trj_batch)
+
+    def adjoint_grid_only(self, ksp: torch.Tensor, trj: torch.Tensor) -> torch.Tensor:
+        """KB gridding onto a centred grid, / w^d."""
+        _lib.require_device(ksp, trj)
+        d = trj.shape[-1]
+        N = trj.shape[0]
+        trj_batch = trj.shape[1:-1]
+        M = int(math.prod(trj_batch))
+        lead = ksp.shape[1: ksp.ndim - len(trj_batch)]
+        B = int(math.prod(lead))
+        G = self.os_shape()
+        out = torch.zeros((N, B, *G), dtype=complex_dtype, device=ksp.device)
+        src = ksp.to(complex_dtype).reshape(N, B, M)
+        for i in range(N):
+            plan = self.plan(trj[i], grid_size=G, centred=True)
+            ops.gridding(src[i], plan, out[i], out_scale=1.0 / self.width ** d)
+        return out.reshape(N, *lead, *G)
+
+    def adjoint_iFT_only(self, ksp_os: torch.Tensor) -> torch.Tensor:
+        """centred iFFT (norm=None) -> crop -> * prod(G)/sqrt(prod N) -> apodize."""
+        _lib.require_device(ksp_os)
+        d = len(self.im_size)
+        G = tuple(ksp_os.shape[-d:])
+        lead = ksp_os.shape[:-d]
+        g = fft_torch.ifftshift(ksp_os.to(complex_dtype), dim=tuple(range(-d, 0))).contiguous()
+        g = g.reshape(-1, *G)
+        _ifftn_(g, d)
+        img = ops.crop_reduce(g, self.im_size, G, g.shape[0], apod=self.apod_table(ksp_os.device),
+                              scale=1.0 / math.sqrt(math.prod(self.im_size)))
+        return img.reshape(*lead, *self.im_size)
+
+    # -- fused transforms ------------------------------------------------
+    def forward(self, img: torch.Tensor, trj: torch.Tensor) -> torch.Tensor:
+        """img (N, *img_batch, *im_size), trj (N, *trj_batch, d) -> ksp (N, *img_batch, *trj_batch)."""
+        _lib.require_device(img, trj)
+        d = len(self.im_size)
+        N = trj.shape[0]
+        img_batch = img.shape[1:-d]
+        B = int(math.prod(img_batch))
+        trj_batch = trj.shape[1:-1]
+        M = int(math.prod(trj_batch))
+        G = self.os_shape()
+        apod = self.apod_table(img.device)
+        out = torch.empty((N, B, M), dtype=complex_dtype, device=img.device)
+        x = img.to(complex_dtype).reshape(img.shape[0], B, *self.im_size)
+        for n in range(N):
+            plan = self.plan(trj[n])
+            g = ops.pad_expand(x[n if x.shape[0] > 1 else 0], self.im_size, G, apod=apod,
+                               scale=1.0 / math.sqrt(math.prod(self.im_size)))
+            _fftn_(g, d)
+            ops.interp(g, plan, out_scale=1.0 / self.width ** d, out=out[n])
+        return out.reshape(N, *img_batch, *trj_batch)
+
+    def adjoint(self, ksp: torch.Tensor, trj: torch.Tensor) -> torch.Tensor:
+        """ksp (N, *ksp_batch, *trj_batch), trj (N, *trj_batch, d) -> img (N, *ksp_batch, *im_size)."""
+        _lib.require_device(ksp, trj)
+        d = len(self.im_size)
+        N = trj.shape[0]
+        trj_batch = trj.shape[1:-1]
+        M = int(math.prod(trj_batch))
+        ksp_batch = ksp.shape[1: ksp.ndim - len(trj_batch)]
+        B = int(math.prod(ksp_batch))
+        G = self.os_shape()
+        apod = self.apod_table(ksp.device)
+        out = torch.empty((N, B, *self.im_size), dtype=complex_dtype, device=ksp.device)
+        y = ksp.to(complex_dtype).reshape(ksp.shape[0], B, M)
+        scale = 1.0 / (self.width ** d * math.sqrt(math.prod(self.im_size)))
+        for n in range(N):
+            plan = self.plan(trj[n])
+            g = torch.zeros((B, *G), dtype=complex_dtype, device=ksp.device)
+            ops.gridding(y[n if y.shape[0] > 1 else 0], plan, g)
+            _ifftn_(g, d)
+            ops.crop_reduce(g, self.im_size, G, B, apod=apod, scale=scale, out=out[n])
+        return out.reshape(N, *ksp_batch, *self.im_size)
+
+    def calc_teoplitz_kernels(self, trj: torch.Tensor, weights: Optional[torch.Tensor] = None,
+                              os_factor: Optional[float] = 2.0):
+        """Centred PSF kernels (N, *round(os_factor * im_size)) (fourier.py:552-582)."""
+        im_size_os = tuple(round(i * os_factor) for i in self.im_size)
+        nufft_os = hip_nufft(im_size=im_size_os, oversamp=self.oversamp, width=self.width,
+                             samples_per_item=self.samples_per_item)
+        return calc_toep_kernel_helper(nufft_os.adjoint, trj * os_factor, weights) * (os_factor ** len(self.im_size))
+
+
+# The reference class name, so code written against mr_recon.fourier.triton_nufft
+# switches by changing the import only.
+triton_nufft = hip_nufft
+
+__all__ = ["fft", "ifft", "calc_toep_kernel_helper", "NUFFT", "hip_nufft", "triton_nufft", "kb_beta",
+           "toeplitz_unshifted", "batch_iterator"]

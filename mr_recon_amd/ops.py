@@ -1,0 +1,166 @@
+"""Thin tensor-level wrappers over the C-ABI kernels.
+
+All functions take device tensors, launch on PyTorch's current HIP stream and
+return without synchronising. Shapes are validated here so that every launch
+matches what the kernel and its grid assume.
+"""
+
+from __future__ import annotations
+
+import math
+
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr, stream_ptr
+from .plan import SamplePlan
+
+
+def _c64(t: torch.Tensor, name: str) -> torch.Tensor:
+    if t.dtype != torch.complex64:
+        raise TypeError(f"{name} must be complex64, got {t.dtype}")
+    if not t.is_contiguous():
+        t = t.contiguous()
+    return t
+
+
+def interp(grid: torch.Tensor, plan: SamplePlan, out_scale: float = 1.0, out: torch.Tensor | None = None,
+           geom: torch.Tensor | None = None) -> torch.Tensor:
+    """Gather: ``grid`` (B, *G) complex64 -> samples (B, M) complex64."""
+    _lib.require_device(grid)
+    grid = _c64(grid, "grid")
+    D = plan.ndim
+    if tuple(grid.shape[-D:]) != plan.grid_size:
+        raise ValueError(f"grid shape {tuple(grid.shape[-D:])} does not match plan {plan.grid_size}")
+    B = int(math.prod(grid.shape[:-D]))
+    if out is None:
+        out = torch.empty((B, plan.npts), dtype=torch.complex64, device=grid.device)
+    elif out.numel() != B * plan.npts or not out.is_contiguous():
+        raise ValueError("interp: out must be contiguous with B*M elements")
+    g = geom if geom is not None else plan.geom(B, plan.npts, out_scale)
+    check(lib().mra_interp(D, plan.P, ptr(grid), ptr(out), ptr(plan.tsort), ptr(plan.perm), ptr(g),
+                           ptr(plan.items), plan.max_items, stream_ptr(grid.device)))
+    return out
+
+
+def gridding(ksp: torch.Tensor, plan: SamplePlan, out: torch.Tensor, dcf: torch.Tensor | None = None,
+             out_scale: float = 1.0) -> torch.Tensor:
+    """Scatter-accumulate: samples (B, M) complex64 -> ``out`` (B, *G), added in place.
+
+    ``out`` must be zeroed by the caller when a fresh grid is wanted.
+    ``dcf`` (M,) float32 multiplies each sample first.
+    """
+    _lib.require_device(ksp, out, dcf)
+    ksp = _c64(ksp, "ksp")
+    D = plan.ndim
+    if out.dtype != torch.complex64 or not out.is_contiguous():
+        raise TypeError("gridding output must be a contiguous complex64 tensor")
+    if tuple(out.shape[-D:]) != plan.grid_size:
+        raise ValueError(f"output grid {tuple(out.shape[-D:])} does not match plan {plan.grid_size}")
+    B = int(math.prod(out.shape[:-D]))
+    if ksp.numel() != B * plan.npts:
+        raise ValueError(f"gridding: ksp has {ksp.numel()} elements, expected {B}x{plan.npts}")
+    if dcf is not None:
+        if dcf.dtype != torch.float32 or dcf.numel() != plan.npts:
+            raise ValueError("dcf must be float32 with one entry per sample")
+        dcf = dcf.contiguous()
+    g = plan.geom(B, plan.npts, out_scale)
+    check(lib().mra_grid(D, plan.P, ptr(out), ptr(ksp), ptr(dcf), ptr(plan.tsort), ptr(plan.perm), ptr(g),
+                         ptr(plan.items), plan.max_items, stream_ptr(out.device)))
+    return out
+
+
+def interp_grad(grid: torch.Tensor, a: torch.Tensor, plan: SamplePlan, out_scale: float = 1.0) -> torch.Tensor:
+    """d/dt of Re<a, interp(grid, t)> per sample: returns (M, D) float32 in grid units.
+
+    gt[m, d] = sum_b Re( conj(a[b, m]) * sum_p dW_p/dt_d * grid[b, g_p] )
+    """
+    _lib.require_device(grid, a)
+    grid = _c64(grid, "grid")
+    a = _c64(a, "a")
+    D = plan.ndim
+    B = int(math.prod(grid.shape[:-D]))
+    if a.numel() != B * plan.npts:
+        raise ValueError("interp_grad: a must hold B*M samples")
+    gt = torch.empty((plan.npts, D), dtype=torch.float32, device=grid.device)
+    g = plan.geom(B, plan.npts, out_scale)
+    check(lib().mra_interp_grad(D, plan.P, ptr(grid), ptr(a), ptr(gt), ptr(plan.tsort), ptr(plan.perm), ptr(g),
+                                ptr(plan.items), plan.max_items, stream_ptr(grid.device)))
+    return gt
+
+
+def pad_expand(x: torch.Tensor, im_size, grid_size, smaps: torch.Tensor | None = None,
+               apod: torch.Tensor | None = None, scale: float = 1.0, center=None,
+               out: torch.Tensor | None = None) -> torch.Tensor:
+    """x (Bx, *N) [, smaps (C, *N)] -> grid (Bx, C, *G) in the unshifted-FFT layout.
+
+    Pixel n lands at grid index (n - c) mod G with c = N//2 unless ``center`` is
+    given. ``apod`` is the concatenation of the per-dim apodization tables.
+    """
+    _lib.require_device(x, smaps, apod)
+    D = len(im_size)
+    x = _c64(x, "x")
+    if tuple(x.shape[-D:]) != tuple(im_size):
+        raise ValueError(f"image shape {tuple(x.shape[-D:])} != {tuple(im_size)}")
+    nx = int(math.prod(x.shape[:-D]))
+    npix = int(math.prod(im_size))
+    ns = 1
+    if smaps is not None:
+        smaps = _c64(smaps, "smaps")
+        if tuple(smaps.shape[-D:]) != tuple(im_size):
+            raise ValueError("maps must have the image shape")
+        ns = int(math.prod(smaps.shape[:-D]))
+    if apod is not None and (apod.dtype != torch.float32 or apod.numel() != sum(im_size)):
+        raise ValueError("apod must be float32 with sum(im_size) entries")
+    shape = (nx, ns, *grid_size) if smaps is not None else (nx, *grid_size)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.complex64, device=x.device)
+    elif out.numel() != math.prod(shape) or out.dtype != torch.complex64 or not out.is_contiguous():
+        raise ValueError("pad_expand: bad output tensor")
+    cen = None if center is None else _lib.ints(center)
+    check(lib().mra_pad_expand(D, _lib.ints(im_size), _lib.ints(grid_size), cen, ptr(x), nx, npix, ptr(smaps), ns,
+                               npix, ptr(apod), float(scale), ptr(out), stream_ptr(x.device)))
+    return out
+
+
+def crop_reduce(grid: torch.Tensor, im_size, grid_size, nx: int, smaps: torch.Tensor | None = None,
+                apod: torch.Tensor | None = None, scale: float = 1.0, out: torch.Tensor | None = None,
+                accumulate: bool = False, roll_out: bool = False, conj_maps: bool = True,
+                center=None) -> torch.Tensor:
+    """grid (nx, C, *G) -> image (nx, *N): crop, deapodize and (with maps) sum_c conj(S_c) * grid_c."""
+    _lib.require_device(grid, smaps, apod, out)
+    D = len(im_size)
+    grid = _c64(grid, "grid")
+    npix = int(math.prod(im_size))
+    ns = 1
+    if smaps is not None:
+        smaps = _c64(smaps, "smaps")
+        ns = int(math.prod(smaps.shape[:-D]))
+    if grid.numel() != nx * ns * math.prod(grid_size):
+        raise ValueError(f"crop_reduce: grid has {grid.numel()} elements, expected {nx}x{ns}x{tuple(grid_size)}")
+    if apod is not None and (apod.dtype != torch.float32 or apod.numel() != sum(im_size)):
+        raise ValueError("apod must be float32 with sum(im_size) entries")
+    if out is None:
+        if accumulate:
+            raise ValueError("accumulate needs an output tensor")
+        out = torch.empty((nx, *im_size), dtype=torch.complex64, device=grid.device)
+    elif out.numel() != nx * npix or out.dtype != torch.complex64 or not out.is_contiguous():
+        raise ValueError("crop_reduce: bad output tensor")
+    cen = None if center is None else _lib.ints(center)
+    check(lib().mra_crop_reduce(D, _lib.ints(im_size), _lib.ints(grid_size), cen, ptr(grid), nx, ptr(smaps), ns,
+                                npix, int(conj_maps), ptr(apod), float(scale), int(accumulate), int(roll_out),
+                                ptr(out), npix, stream_ptr(grid.device)))
+    return out
+
+
+def cmul_bcast(data: torch.Tensor, kern: torch.Tensor) -> torch.Tensor:
+    """In place: data (B, *S) *= kern (*S), complex64."""
+    _lib.require_device(data, kern)
+    if not data.is_contiguous() or data.dtype != torch.complex64:
+        raise ValueError("cmul_bcast: data must be contiguous complex64")
+    kern = _c64(kern, "kern")
+    n = kern.numel()
+    if data.numel() % n:
+        raise ValueError("cmul_bcast: data size is not a multiple of the kernel size")
+    check(lib().mra_cmul_bcast(ptr(data), ptr(kern), data.numel() // n, n, stream_ptr(data.device)))
+    return data

@@ -1,0 +1,154 @@
+"""Sample plans: tile binning of one trajectory against one oversampled grid.
+
+A plan is built once per (trajectory, grid geometry, kernel width) and reused by
+every interp / gridding / gradient call on that trajectory, so the CG loop pays
+for binning once. Building it runs three device passes (per-chunk tile
+histograms, a scan, a stable scatter) and two small ones for the work items;
+there is no host synchronisation, so plans can also be rebuilt every step when
+the trajectory itself is being optimised.
+
+Coordinate handling is bit-exact with the reference:
+  * ``triton_nufft._scale_trj`` (fourier.py:420-430): t = fl32(fl32(k * fl32(G/N)) + G//2)
+  * ``prep_ungrid_shapes`` / ``prep_grid_shapes`` (ungrid.py:864-865, grid.py:516-517):
+    t = torch.remainder(t, G)
+  * ``get_neighborhood`` (kernels.py:531-536): lower = ceil(t - w/2)
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr, stream_ptr
+
+CHUNK_LEN = 4096  # samples per binning chunk (one wave walks one chunk in order)
+
+
+def kernel_points(width: float) -> int:
+    """Grid points per dimension a KB patch of width ``width`` can touch."""
+    return int(math.floor(float(width))) + 1
+
+
+@dataclass(frozen=True)
+class GeomKey:
+    nbatch: int
+    ksp_stride: int
+    out_scale: float
+    grid_stride: int
+
+
+class SamplePlan:
+    """Tile-sorted samples of one trajectory on one grid.
+
+    Parameters
+    ----------
+    coords : (..., D) float32 device tensor, reference units (see ``scale``)
+    grid_size : tuple of D ints, the (oversampled) grid the samples live on
+    width : kernel width in grid points (same in every dim)
+    beta : Kaiser-Bessel shape parameter
+    scale, shift : per-dim fp32 scale and shift applied before wrapping (the
+        NUFFT's ``_scale_trj``); ``None`` means coordinates are already in grid
+        units (the plain ``ungrid``/``grid`` surface)
+    wrap : per-dim integer added to a grid index before it wraps mod G. The
+        NUFFT uses ``-(G//2)`` so kernels address the unshifted FFT output.
+    tile : tile edge in grid points (default per dimensionality)
+    samples_per_item : max samples per work item (<= 512)
+    """
+
+    def __init__(self, coords: torch.Tensor, grid_size, width: float, beta: float, *, scale=None, shift=None,
+                 wrap=None, tile: int | None = None, samples_per_item: int | None = None):
+        _lib.require_device(coords)
+        L = lib()
+        D = len(grid_size)
+        if coords.shape[-1] != D:
+            raise ValueError(f"coords last dim {coords.shape[-1]} != grid ndim {D}")
+        if D < 1 or D > 3:
+            raise ValueError("the HIP NUFFT supports 1D, 2D and 3D grids")
+        if coords.dtype != torch.float32:
+            raise TypeError(f"coordinates must be float32, got {coords.dtype}")
+        self.device = coords.device
+        self.ndim = D
+        self.grid_size = tuple(int(g) for g in grid_size)
+        self.width = float(width)
+        self.P = kernel_points(width)
+        if not 2 <= self.P <= 9:
+            raise ValueError(f"kernel width {width} unsupported: floor(width)+1 must be in 2..9")
+        self.beta = float(beta)
+        self.coords_shape = tuple(coords.shape[:-1])
+        self.npts = int(math.prod(self.coords_shape))
+        if self.npts <= 0:
+            raise ValueError("trajectory has no samples")
+        self.tile = int(tile or L.mra_default_tile(D))
+        self.ntile = [(g + self.tile - 1) // self.tile for g in self.grid_size]
+        self.ntiles = int(math.prod(self.ntile))
+        spi_max = L.mra_samples_per_item()
+        self.S = int(samples_per_item or spi_max)
+        if not 1 <= self.S <= spi_max:
+            raise ValueError(f"samples_per_item must be in 1..{spi_max}")
+        self.wrap = tuple(int(w) for w in (wrap if wrap is not None else (0,) * D))
+
+        flat = coords.reshape(-1, D).contiguous()
+        dev = self.device
+        nchunks = C.c_int()
+        hist_elems = C.c_longlong()
+        scan_elems = C.c_longlong()
+        check(L.mra_bin_sizes(self.npts, self.ntiles, CHUNK_LEN, C.byref(nchunks), C.byref(hist_elems),
+                              C.byref(scan_elems)))
+        hist = torch.empty(hist_elems.value, dtype=torch.int32, device=dev)
+        scanned = torch.empty(hist_elems.value + 1, dtype=torch.int32, device=dev)
+        scan_ws = torch.empty(scan_elems.value, dtype=torch.int32, device=dev)
+        self.tsort = torch.empty(self.npts * D, dtype=torch.float32, device=dev)
+        self.perm = torch.empty(self.npts, dtype=torch.int32, device=dev)
+        apply = scale is not None
+        sc = _lib.floats(scale if apply else [1.0] * D)
+        sh = _lib.floats(shift if shift is not None else [0.0] * D)
+        st = stream_ptr(dev)
+        check(L.mra_bin_build(D, ptr(flat), self.npts, _lib.ints(self.grid_size), sc, sh, int(apply),
+                              self.width, self.tile, _lib.ints(self.wrap), CHUNK_LEN, ptr(hist), ptr(scanned),
+                              ptr(scan_ws), ptr(self.tsort), ptr(self.perm), st))
+        nkeys = self.ntiles + 1
+        self.max_items = nkeys + (self.npts + self.S - 1) // self.S
+        self.items = torch.empty((self.max_items, 4), dtype=torch.int32, device=dev)
+        nitems_ws = torch.empty(nkeys + 1, dtype=torch.int32, device=dev)
+        item_off = torch.empty(nkeys + 1, dtype=torch.int32, device=dev)
+        scan_ws2 = torch.empty(L.mra_scan_workspace(nkeys + 1), dtype=torch.int32, device=dev)
+        self.tile_start = torch.empty(nkeys + 1, dtype=torch.int32, device=dev)
+        check(L.mra_bin_items(ptr(scanned), nchunks.value, self.ntiles, self.S, 0, ptr(nitems_ws), ptr(item_off),
+                              ptr(scan_ws2), ptr(self.items), self.max_items, ptr(self.tile_start), st))
+        # The workspaces go back to the caching allocator here; PyTorch only
+        # hands them out again to work ordered after these kernels on this stream.
+        del hist, scanned, scan_ws, nitems_ws, item_off, scan_ws2, flat
+        self._geoms: dict[GeomKey, torch.Tensor] = {}
+
+    def geom(self, nbatch: int, ksp_stride: int | None = None, out_scale: float = 1.0,
+             grid_stride: int | None = None) -> torch.Tensor:
+        """Device Geom table (one entry) for a call on ``nbatch`` grids."""
+        gstride = int(grid_stride if grid_stride is not None else math.prod(self.grid_size))
+        kstride = int(ksp_stride if ksp_stride is not None else self.npts)
+        key = GeomKey(int(nbatch), kstride, float(out_scale), gstride)
+        g = self._geoms.get(key)
+        if g is None:
+            g = make_geoms([dict(grid_off=0, grid_stride=gstride, ksp_off=0, ksp_stride=kstride, samp_off=0,
+                                 nbatch=int(nbatch), npts=self.npts, grid_size=self.grid_size, tile=self.tile,
+                                 wrap=self.wrap, beta=self.beta, width=self.width, out_scale=out_scale)],
+                           self.device)
+            self._geoms[key] = g
+        return g
+
+
+def make_geoms(entries: list[dict], device) -> torch.Tensor:
+    """Pack Geom records (see common.h) into a device byte tensor."""
+    L = lib()
+    gsz = L.mra_geom_size()
+    buf = (C.c_ubyte * (gsz * len(entries)))()
+    for i, e in enumerate(entries):
+        D = len(e["grid_size"])
+        check(L.mra_geom_fill(C.addressof(buf) + i * gsz, e["grid_off"], e["grid_stride"], e["ksp_off"],
+                              e["ksp_stride"], e["samp_off"], e["nbatch"], e["npts"], D, _lib.ints(e["grid_size"]),
+                              e["tile"], _lib.ints(e["wrap"]), e["beta"], e["width"], e["out_scale"]))
+    host = torch.frombuffer(bytearray(bytes(buf)), dtype=torch.uint8)
+    return host.to(device)

@@ -1,0 +1,109 @@
+"""SENSE linop, CG, power method and CG-SENSE driver restated on CPU torch.
+
+  ``SenseRef``              linops.py:1041-1308 (coil batch 1, L = 1 segment)
+  ``conjugate_gradient``    algs.py:779-872
+  ``power_method_operator`` algs.py:364-399
+  ``cg_sense_recon``        recons.py:77-156
+"""
+
+from __future__ import annotations
+
+import torch
+
+from .nufft_ref import PadLastRef, fft_c, ifft_c
+
+
+class SenseRef:
+    def __init__(self, trj, mps, dcf=None, nufft=None, use_toeplitz=False):
+        self.im_size = tuple(mps.shape[1:])
+        self.trj_size = tuple(trj.shape[:-1])
+        self.nufft = nufft
+        self.cdt = nufft.cdtype
+        self.mps = mps.to(self.cdt)
+        self.trj = trj.to(torch.float32)
+        self.dcf = torch.ones(self.trj_size, dtype=torch.float32) if dcf is None else dcf.to(torch.float32)
+        self.use_toeplitz = use_toeplitz
+        if use_toeplitz:
+            self.toep_kerns = nufft.calc_teoplitz_kernels(self.trj[None], self.dcf[None])[0]
+        self.ishape = self.im_size
+
+    def forward(self, img):
+        nc = self.mps.shape[0]
+        ksp = torch.zeros((nc, *self.trj_size), dtype=self.cdt)
+        for c in range(nc):
+            sx = self.mps[c:c + 1] * img
+            ksp[c:c + 1] += self.nufft.forward(sx[None, :, None], self.trj[None])[0][:, 0]
+        return ksp
+
+    def adjoint(self, ksp):
+        nc = self.mps.shape[0]
+        img = torch.zeros(self.im_size, dtype=self.cdt)
+        for c in range(nc):
+            y = ksp[c:c + 1] * self.dcf[None]
+            f = self.nufft.adjoint(y[None, :, None], self.trj[None])[0][:, 0]
+            img += (f * self.mps[c:c + 1].conj()).sum(0)
+        return img
+
+    def normal(self, img):
+        if not self.use_toeplitz:
+            return self.adjoint(self.forward(img))
+        d = len(self.im_size)
+        pad = PadLastRef(self.toep_kerns.shape[-d:], self.im_size)
+        dims = tuple(range(-d, 0))
+        out = torch.zeros_like(img, dtype=self.cdt)
+        for c in range(self.mps.shape[0]):
+            f = fft_c(pad.forward((self.mps[c] * img)[None]), dims)
+            f = ifft_c(f * self.toep_kerns, dims)
+            out += (pad.adjoint(f)[0] * self.mps[c].conj())
+        return out
+
+
+def conjugate_gradient(AHA, AHb, num_iters=10, lamda_l2=0.0, tolerance=1e-8, return_resids=False):
+    """algs.py:779-872 with P = identity and weights = None."""
+    def aha_w(x):
+        return AHA(x) + lamda_l2 * x
+
+    x0 = AHb.clone()
+    if num_iters == 0:
+        return x0
+    r = AHb - aha_w(x0)
+    z = r
+    p = z.clone()
+    resids = []
+    for _ in range(num_iters):
+        Ap = aha_w(p)
+        pAp = torch.real(torch.sum(p.conj() * Ap)).item()
+        rz = torch.real(torch.sum(r.conj() * z))
+        alpha = rz / pAp
+        x0 = x0 + alpha * p
+        r = r - alpha * Ap
+        rnrm = torch.norm(r)
+        resids.append(rnrm.item())
+        if rnrm < tolerance:
+            break
+        z = r
+        beta = torch.real(torch.sum(r.conj() * z)) / rz
+        p = z + beta * p
+    return (resids, x0) if return_resids else x0
+
+
+def power_method_operator(A, x0, num_iter=15):
+    """algs.py:364-399."""
+    for _ in range(num_iter):
+        z = A(x0)
+        ll = torch.norm(z)
+        x0 = z / ll
+    return x0, ll.item()
+
+
+def cg_sense_recon(A, ksp, max_iter=15, lamda_l2=0.0, max_eigen=None, tolerance=1e-8):
+    """recons.py:77-156."""
+    if max_eigen is None:
+        x0 = torch.randn(A.ishape, dtype=A.cdt)
+        _, max_eigen = power_method_operator(A.normal, x0)
+        max_eigen *= 1.01
+    AHb = A.adjoint(ksp.to(A.cdt)) / (max_eigen ** 0.5)
+    if max_iter == 0:
+        return AHb
+    return conjugate_gradient(lambda x: A.normal(x) / max_eigen, AHb, num_iters=max_iter, lamda_l2=lamda_l2,
+                              tolerance=tolerance)

@@ -1,0 +1,171 @@
+"""GPU parity: binning plans (bit-exact), KB interpolation and gridding vs the CPU oracle."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import cdot, edge_coords, rel_l2
+from oracle import bins as obins
+from oracle import interp_ref
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5  # relative L2, complex64 (fp32 accumulation-order differences only)
+
+
+def _case(D, G, width, n, seed):
+    k = edge_coords((G,) * D, n, seed=seed, width=width)
+    return torch.from_numpy(k)
+
+
+@pytest.mark.parametrize("D,G,width", [(1, 96, 4.0), (2, 64, 4.0), (2, 80, 6.0), (3, 24, 4.0), (3, 20, 6.0),
+                                       (2, 40, 3.5), (3, 16, 8.0)])
+@pytest.mark.parametrize("wrapoff", [0, -1])
+def test_bin_plan_bit_exact(dev, D, G, width, wrapoff):
+    from mr_recon_amd.plan import SamplePlan
+
+    k = _case(D, G, width, 3000, seed=D * 10 + int(width))
+    grid_size = (G,) * D
+    wrap = tuple(wrapoff * (g // 2) for g in grid_size)
+    plan = SamplePlan(k.to(dev), grid_size, width, 2.0, wrap=wrap)
+    ts, perm, start, _ = obins.bin_plan(k.numpy(), grid_size, width, plan.tile, wrap=wrap)
+    got_t = plan.tsort.cpu().numpy()
+    assert np.array_equal(plan.perm.cpu().numpy(), perm)
+    assert np.array_equal(got_t.view(np.int32), ts.view(np.int32))
+    assert np.array_equal(plan.tile_start.cpu().numpy(), start)
+
+
+@pytest.mark.parametrize("D,N", [(2, 64), (3, 20)])
+def test_bin_plan_scaled_bit_exact(dev, D, N):
+    """The NUFFT scaling t = fl32(fl32(k * fl32(G/N)) + G//2), including non-dyadic G/N."""
+    from mr_recon_amd.plan import SamplePlan
+
+    for G in (math.ceil(1.25 * N), math.ceil(1.3 * N) + 1, 2 * N):
+        rng = np.random.default_rng(G)
+        k = ((rng.random((4000, D)) - 0.5) * N).astype(np.float32)
+        k[:20] = np.float32(-N / 2)
+        k[20:40] = np.float32(N / 2)
+        sc = [float(np.float32(G / N))] * D
+        sh = [float(G // 2)] * D
+        grid = (G,) * D
+        plan = SamplePlan(torch.from_numpy(k).to(dev), grid, 4.0, 5.0, scale=sc, shift=sh,
+                          wrap=tuple(-(g // 2) for g in grid))
+        ts, perm, start, _ = obins.bin_plan(k, grid, 4.0, plan.tile, wrap=tuple(-(g // 2) for g in grid), scale=sc,
+                                            shift=sh)
+        assert np.array_equal(plan.perm.cpu().numpy(), perm)
+        assert np.array_equal(plan.tsort.cpu().numpy().view(np.int32), ts.view(np.int32))
+        # the oracle's scaling equals what the reference computes with torch ops
+        t_ref = torch.from_numpy(k).clone()
+        for d in range(D):
+            t_ref[:, d] *= G / N
+            t_ref[:, d] += G // 2
+        t_ref = torch.remainder(t_ref, torch.tensor(grid))
+        assert np.array_equal(t_ref.numpy()[perm].reshape(-1).view(np.int32), ts.view(np.int32))
+
+
+@pytest.mark.parametrize("D,G,width,B", [(1, 128, 4.0, 3), (2, 48, 4.0, 2), (2, 64, 6.0, 4), (3, 20, 4.0, 3),
+                                         (3, 16, 6.0, 2), (2, 32, 3.0, 1), (3, 12, 8.0, 1), (2, 40, 2.5, 2)])
+def test_ungrid_matches_oracle(dev, D, G, width, B):
+    from mr_recon_amd.hip_interp import ungrid
+
+    g = torch.Generator().manual_seed(D * 100 + G)
+    vals = torch.randn((B, *(G,) * D), dtype=torch.complex64, generator=g)
+    k = _case(D, G, width, 2500, seed=G)
+    beta = 7.3
+    out = ungrid(vals.to(dev), k.to(dev), width, kernel_params={"beta": beta})
+    ref = interp_ref.ungrid(vals, k, width, beta=beta)
+    assert out.shape == ref.shape
+    assert rel_l2(out, ref) < TOL
+
+
+@pytest.mark.parametrize("D,G,width,B", [(1, 128, 4.0, 2), (2, 48, 4.0, 2), (2, 64, 6.0, 3), (3, 20, 4.0, 2),
+                                         (3, 16, 6.0, 1), (2, 32, 3.0, 1)])
+def test_grid_matches_oracle(dev, D, G, width, B):
+    from mr_recon_amd.hip_interp import grid
+
+    k = _case(D, G, width, 2500, seed=G + 1)
+    g = torch.Generator().manual_seed(7)
+    vals = torch.randn((B, k.shape[0]), dtype=torch.complex64, generator=g)
+    beta = 6.1
+    out = grid(vals.to(dev), k.to(dev), (G,) * D, width, kernel_params={"beta": beta})
+    ref = interp_ref.grid(vals, k, (G,) * D, width, beta=beta)
+    assert out.shape == ref.shape
+    assert rel_l2(out, ref) < TOL
+
+
+def test_ungrid_bruteforce_tiny(dev):
+    """Tiny grids (G smaller than the patch) against the pure-Python definition."""
+    from mr_recon_amd.hip_interp import ungrid
+
+    for D, G, w in [(1, 5, 4.0), (2, 6, 4.0), (3, 5, 4.0), (2, 3, 6.0)]:
+        g = torch.Generator().manual_seed(G)
+        vals = torch.randn((2, *(G,) * D), dtype=torch.complex64, generator=g)
+        k = (torch.rand((40, D), generator=g) - 0.3) * 2 * G
+        out = ungrid(vals.to(dev), k.to(dev), w, kernel_params={"beta": 4.0}).cpu()
+        ref = interp_ref.ungrid_bruteforce(vals, k, w, 4.0)
+        assert rel_l2(out, torch.from_numpy(ref)) < TOL
+
+
+def test_adjointness(dev):
+    from mr_recon_amd.hip_interp import grid, ungrid
+
+    for D, G in [(2, 50), (3, 18)]:
+        g = torch.Generator().manual_seed(3)
+        x = torch.randn((2, *(G,) * D), dtype=torch.complex64, generator=g)
+        k = (torch.rand((3000, D), generator=g) * 2 - 0.5) * G
+        y = torch.randn((2, 3000), dtype=torch.complex64, generator=g)
+        ax = ungrid(x.to(dev), k.to(dev), 4.0, kernel_params={"beta": 6.0})
+        aty = grid(y.to(dev), k.to(dev), (G,) * D, 4.0, kernel_params={"beta": 6.0})
+        lhs, rhs = cdot(y, ax), cdot(aty, x)
+        assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+def test_nonfinite_coords_give_zero(dev):
+    from mr_recon_amd.hip_interp import grid, ungrid
+
+    vals = torch.randn((2, 32, 32), dtype=torch.complex64)
+    k = torch.rand((100, 2)) * 32
+    k[5, 0] = float("nan")
+    k[17, 1] = float("inf")
+    out = ungrid(vals.to(dev), k.to(dev), 4.0).cpu()
+    assert torch.all(out[:, 5] == 0) and torch.all(out[:, 17] == 0)
+    ok = torch.ones(100, dtype=torch.bool)
+    ok[[5, 17]] = False
+    ref = interp_ref.ungrid(vals, k[ok], 4.0)
+    assert rel_l2(out[:, ok], ref) < TOL
+    y = torch.randn((2, 100), dtype=torch.complex64)
+    gr = grid(y.to(dev), k.to(dev), (32, 32), 4.0).cpu()
+    ref_g = interp_ref.grid(y[:, ok], k[ok], (32, 32), 4.0)
+    assert rel_l2(gr, ref_g) < TOL
+
+
+def test_real_and_batched_shapes(dev):
+    from mr_recon_amd.hip_interp import grid, ungrid
+
+    vals = torch.randn((2, 3, 40, 40))
+    k = (torch.rand((7, 11, 2)) - 0.5) * 40
+    out = ungrid(vals.to(dev), k.to(dev), 4.0)
+    assert out.shape == (2, 3, 7, 11) and not torch.is_complex(out)
+    ref = interp_ref.ungrid(vals.to(torch.complex64), k, 4.0).real
+    assert rel_l2(out, ref) < TOL
+    y = torch.randn((2, 3, 7, 11))
+    gr = grid(y.to(dev), k.to(dev), (40, 40), 4.0)
+    assert gr.shape == (2, 3, 40, 40)
+    ref = interp_ref.grid(y.to(torch.complex64), k, (40, 40), 4.0).real
+    assert rel_l2(gr, ref) < TOL
+
+
+def test_dense_hot_tile_split(dev):
+    """Many samples in one tile (radial centre): items split the tile; results unchanged."""
+    from mr_recon_amd.hip_interp import grid, ungrid
+
+    g = torch.Generator().manual_seed(11)
+    k = 30 + torch.rand((20000, 2), generator=g) * 3  # all inside one 32-tile
+    vals = torch.randn((3, 64, 64), dtype=torch.complex64, generator=g)
+    out = ungrid(vals.to(dev), k.to(dev), 6.0, kernel_params={"beta": 9.0})
+    assert rel_l2(out, interp_ref.ungrid(vals, k, 6.0, beta=9.0)) < TOL
+    y = torch.randn((3, 20000), dtype=torch.complex64, generator=g)
+    gr = grid(y.to(dev), k.to(dev), (64, 64), 6.0, kernel_params={"beta": 9.0})
+    assert rel_l2(gr, interp_ref.grid(y, k, (64, 64), 6.0, beta=9.0)) < TOL

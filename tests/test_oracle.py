@@ -1,0 +1,194 @@
+"""Pins the CPU oracle (no GPU): closed forms, library routines, brute force, identities."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+from scipy.special import i0
+
+from helpers import cdot, edge_coords, rel_l2
+from mr_recon_amd import synth
+from oracle import bins as obins
+from oracle import interp_ref, kb, ndft
+from oracle.nufft_ref import TritonNufftRef, fft_c, resize
+from oracle.sense_ref import SenseRef, conjugate_gradient
+
+
+def test_i0_polynomial_vs_scipy():
+    x = np.linspace(0, 30, 20001)
+    rel = np.abs(kb.i0_as_np(x) - i0(x)) / i0(x)
+    assert rel.max() < 2.5e-7  # A&S 9.8.1 / 9.8.2 stated accuracy
+
+
+def test_kb_torch_matches_numpy():
+    u = torch.linspace(0, 1, 1001, dtype=torch.float64)
+    for beta in (2.0, 6.99, 13.9):
+        a = kb.kaiser_bessel_torch(u, beta).numpy()
+        b = kb.i0_as_np(beta * np.sqrt(1 - u.numpy() ** 2))
+        assert np.allclose(a, b, rtol=1e-12)
+
+
+def test_kb_derivative_vs_central_differences():
+    for beta in (3.0, 6.99, 13.9):
+        u = np.linspace(0.0, 0.999, 400)
+        h = 1e-6
+        f = lambda v: kb.i0_as_np(beta * np.sqrt(1 - v ** 2))
+        fd = (f(u + h) - f(u - h)) / (2 * h)
+        an = kb.di0_du_np(u, beta)
+        assert np.allclose(an, fd, rtol=2e-5, atol=1e-6 * np.abs(fd).max())
+        assert np.isfinite(kb.di0_du_np(np.array([1.0]), beta)).all()
+
+
+def test_wrap_matches_torch_remainder():
+    t = np.concatenate([(np.random.default_rng(0).random(100000) - 0.5) * 1000,
+                        [-1e-30, -1e-8, -0.0, 0.0, -160.0, 160.0, 320.0, -3e-6]]).astype(np.float32)
+    for G in (160, 256, 320, 7):
+        a = obins.wrap_coords(t[:, None], (G,))[:, 0]
+        b = torch.remainder(torch.from_numpy(t), torch.tensor([G])).numpy()
+        assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    # the documented edge case: tiny negatives wrap to exactly G
+    assert obins.wrap_coords(np.array([[-1e-30]], np.float32), (160,))[0, 0] == 160.0
+
+
+def test_scale_matches_torch_inplace_ops():
+    k = ((np.random.default_rng(1).random((50000, 2)) - 0.5) * 150).astype(np.float32)
+    for N, s in [(150, 1.25), (128, 1.25), (100, 2.0), (77, 1.3)]:
+        G = math.ceil(s * N)
+        ref = torch.from_numpy(k).clone()
+        for d in range(2):
+            ref[:, d] *= G / N
+            ref[:, d] += G // 2
+        got = obins.scale_coords(k, [G / N] * 2, [G // 2] * 2)
+        assert np.array_equal(got.view(np.int32), ref.numpy().view(np.int32))
+
+
+def test_lower_corner_vs_bruteforce():
+    """ceil(t - r) is the smallest grid point with g - t >= -r; all in-support points are
+    lower..lower+floor(w)."""
+    rng = np.random.default_rng(2)
+    for w in (3.0, 3.5, 4.0, 6.0, 8.0):
+        t = (rng.random(5000) * 100).astype(np.float32)
+        t[:50] = np.round(t[:50])
+        lo = obins.lower_corner(t, w).astype(np.int64)
+        for ti, li in zip(t[:400], lo[:400]):
+            sup = [g for g in range(int(ti) - 10, int(ti) + 11) if abs(np.float32(g) - ti) <= np.float32(w / 2)]
+            assert sup[0] == li and sup[-1] <= li + int(math.floor(w))
+
+
+def test_bin_plan_is_stable_sort():
+    k = edge_coords((40, 40), 5000, seed=4)
+    ts, perm, start, key = obins.bin_plan(k, (40, 40), 4.0, 8)
+    assert np.all(np.diff(key[perm]) >= 0)
+    for b in range(len(start) - 1):
+        seg = perm[start[b]:start[b + 1]]
+        assert np.all(np.diff(seg) > 0)
+
+
+@pytest.mark.parametrize("D,G,w", [(1, 7, 4.0), (2, 6, 4.0), (2, 9, 6.0), (3, 5, 4.0), (2, 8, 3.5)])
+def test_ungrid_vs_bruteforce(D, G, w):
+    g = torch.Generator().manual_seed(D + G)
+    vals = torch.randn((2, *(G,) * D), dtype=torch.complex64, generator=g)
+    k = (torch.rand((50, D), generator=g) - 0.3) * 2 * G
+    out = interp_ref.ungrid(vals, k, w, beta=5.0)
+    ref = interp_ref.ungrid_bruteforce(vals, k, w, 5.0)
+    assert rel_l2(out, torch.from_numpy(ref)) < 2e-6
+
+
+def test_grid_is_adjoint_of_ungrid():
+    for D, G in [(1, 30), (2, 20), (3, 9)]:
+        g = torch.Generator().manual_seed(D)
+        x = torch.randn((2, *(G,) * D), dtype=torch.complex128, generator=g)
+        k = (torch.rand((300, D), generator=g) - 0.5) * 3 * G
+        y = torch.randn((2, 300), dtype=torch.complex128, generator=g)
+        ax = interp_ref.ungrid(x, k, 4.0, beta=6.0, dtype=torch.float64)
+        aty = interp_ref.grid(y, k, (G,) * D, 4.0, beta=6.0, dtype=torch.float64)
+        assert abs(cdot(y, ax) - cdot(aty, x)) < 1e-10 * abs(cdot(y, ax))
+
+
+def test_ndft_integer_k_is_fft():
+    im = (12, 10)
+    x = synth.random_image(im, seed=0).numpy()
+    kk = np.stack(np.meshgrid(np.arange(-6, 6), np.arange(-5, 5), indexing="ij"), -1).reshape(-1, 2)
+    y = ndft.ndft_forward(x, kk, im).reshape(12, 10)
+    ref = fft_c(torch.from_numpy(x), dim=(-2, -1)).numpy()
+    assert np.allclose(y, ref, atol=1e-10)
+
+
+def test_resize_is_centred():
+    x = torch.arange(5, dtype=torch.float32)
+    assert resize(x, (8,)).tolist() == [0, 0, 0, 1, 2, 3, 4, 0]
+    assert resize(torch.arange(8.0), (5,)).tolist() == [2, 3, 4, 5, 6]
+
+
+@pytest.mark.parametrize("os_,w,bound", [(2.0, 6, 2e-5), (1.25, 4, 1.0e-2), (1.25, 6, 8e-4), (2.0, 4, 1.2e-3)])
+def test_oracle_nufft_vs_ndft(os_, w, bound):
+    im = (32, 32)
+    k = ((np.random.default_rng(0).random((1500, 2)) - 0.5) * 32).astype(np.float32)
+    x = synth.random_image(im, seed=1)
+    nf = TritonNufftRef(im, oversamp=os_, width=w, dtype=torch.float64)
+    y = nf.forward(x[None], torch.from_numpy(k)[None])[0].numpy()
+    ye = ndft.ndft_forward(x.numpy(), k, im)
+    assert np.linalg.norm(y - ye) / np.linalg.norm(ye) < bound
+    z = nf.adjoint(torch.from_numpy(ye)[None], torch.from_numpy(k)[None])[0].numpy()
+    ze = ndft.ndft_adjoint(ye, k, im)
+    assert np.linalg.norm(z - ze) / np.linalg.norm(ze) < bound
+
+
+def test_oracle_nufft_fp32_close_to_fp64():
+    im = (24, 24)
+    trj = synth.radial_2d(20, 48, 24).reshape(1, -1, 2)
+    x = synth.random_image((1, *im), seed=2)
+    a = TritonNufftRef(im, 2.0, 6).forward(x, trj)
+    b = TritonNufftRef(im, 2.0, 6, dtype=torch.float64).forward(x, trj)
+    assert rel_l2(a, b) < 2e-6
+
+
+def test_oracle_adjoint_identity_and_toeplitz():
+    im = (20, 20)
+    trj = synth.spiral_2d(4, 200, 20).reshape(1, -1, 2)
+    nf = TritonNufftRef(im, 2.0, 6, dtype=torch.float64)
+    x = synth.random_image((1, *im), seed=3).to(torch.complex128)
+    y = synth.random_image((1, trj.shape[1]), seed=4).to(torch.complex128)
+    assert abs(cdot(y, nf.forward(x, trj)) - cdot(nf.adjoint(y, trj), x)) < 1e-10 * abs(cdot(y, nf.forward(x, trj)))
+    K = nf.calc_teoplitz_kernels(trj)
+    a = nf.normal_toeplitz(x[:, None], K)[:, 0]
+    b = nf.adjoint(nf.forward(x, trj), trj)
+    assert rel_l2(a, b) < 1e-4
+
+
+def test_oracle_toeplitz_exact_for_ndft():
+    """With the NDFT as the operator, the Toeplitz construction is exact (no KB error)."""
+    im = (10, 10)
+    k = ((np.random.default_rng(5).random((200, 2)) - 0.5) * 10)
+    x = synth.random_image(im, seed=6).numpy().astype(np.complex128)
+    normal = ndft.ndft_adjoint(ndft.ndft_forward(x, k, im), k, im)
+    # PSF on the 2N grid from the NDFT adjoint, then the reference's Toeplitz pipeline
+    P = (20, 20)
+    psf = ndft.ndft_adjoint(np.ones(200), 2 * k, P) * 4
+    K = fft_c(torch.from_numpy(psf), dim=(-2, -1))
+    nf = TritonNufftRef(im, 2.0, 6, dtype=torch.float64)
+    out = nf.normal_toeplitz(torch.from_numpy(x)[None, None], K[None])[0, 0].numpy()
+    assert np.linalg.norm(out - normal) / np.linalg.norm(normal) < 1e-10
+
+
+def test_sense_identities_and_cg():
+    im = (16, 16)
+    trj = synth.radial_2d(16, 32, 16).reshape(-1, 2)
+    mps = synth.coil_maps(4, im, seed=0).to(torch.complex128)
+    nf = TritonNufftRef(im, 2.0, 6, dtype=torch.float64)
+    A = SenseRef(trj, mps, nufft=nf)
+    x = synth.random_image(im, seed=1).to(torch.complex128)
+    y = A.forward(x)
+    # forward of coil c equals the NUFFT of S_c x
+    for c in range(4):
+        yc = nf.forward((mps[c] * x)[None], trj[None])[0]
+        assert rel_l2(y[c], yc) < 1e-12
+    r = synth.random_image(y.shape, seed=2).to(torch.complex128)
+    assert abs(cdot(r, y) - cdot(A.adjoint(r), x)) < 1e-10 * abs(cdot(r, y))
+    # CG solves a small SPD system
+    M = torch.randn(30, 30, dtype=torch.complex128)
+    Hm = M.conj().T @ M + 0.5 * torch.eye(30, dtype=torch.complex128)
+    b = torch.randn(30, dtype=torch.complex128)
+    xs = conjugate_gradient(lambda v: Hm @ v, b, num_iters=200, tolerance=1e-14)
+    assert rel_l2(xs, torch.linalg.solve(Hm, b)) < 1e-9
