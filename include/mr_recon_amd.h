@@ -1,0 +1,110 @@
+/* C-ABI of libmr_recon_amd.so -- MI355X (gfx950) kernels of the mr_recon NUFFT hot path.
+ *
+ * Every entry point takes raw device pointers, sizes and the HIP stream to
+ * launch on (void*, a hipStream_t); it enqueues work and returns without
+ * synchronising. Return value: 0 on success, non-zero on a validation or launch
+ * error, with the message available from mra_last_error(). The library never
+ * allocates or frees device memory: callers pass every buffer, including
+ * workspaces, whose sizes the *_sizes / *_workspace / *_points queries give.
+ *
+ * complex64 buffers are interleaved (re, im) float pairs, the layout of
+ * torch.complex64. References "file:line" are to danielabrahamgit/mr_recon.
+ */
+#ifndef MR_RECON_AMD_H
+#define MR_RECON_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- library ------------------------------------------------------------------------- */
+const char* mra_last_error(void);
+int mra_version(void);
+
+/* ---- geometry records ------------------------------------------------------------------
+ * A Geom record describes one problem (grid size, batch, sample count, kernel) for
+ * the interp / gridding / gradient launches; a table of them in device memory lets
+ * one launch serve many uneven problems. */
+int mra_geom_size(void);
+int mra_default_tile(int ndim);
+long long mra_subgrid_points(int ndim, int P);
+int mra_max_coil_chunk(int ndim, int P);
+int mra_grad_samples_per_block(void);
+int mra_geom_fill(void* dst_host, long long grid_off, long long grid_stride, long long ksp_off,
+                  long long ksp_stride, long long samp_off, int nbatch, int npts, int ndim, const int* G,
+                  int tile, const int* wrap, float beta, float width, float out_scale, int item_base,
+                  long long scratch_off, int tile_items_off);
+
+/* ---- sample plans (binning) -------------------------------------------------------------
+ * Replaces the per-call coordinate handling of the reference:
+ *   triton_nufft._scale_trj          fourier.py:420-430   t = fl32(fl32(k*fl32(G/N)) + G//2)
+ *   prep_ungrid_shapes / prep_grid   ungrid.py:851-894, grid.py:508-553   t = remainder(t, G)
+ *   get_neighborhood                 triton_interp/kernels.py:531-536     lower = ceil(t - w/2)
+ * bit for bit, then sorts samples (stably) by the tile of their patch's lower corner. */
+long long mra_scan_workspace(long long n);
+int mra_scan_excl_i32(const int* in, long long n, int* out, int* ws, void* stream);
+int mra_bin_sizes(long long npts, int ntiles, int chunk_len, int* nchunks, long long* hist_elems,
+                  long long* scan_ws_elems);
+int mra_bin_build(int ndim, const float* coords, long long npts, const int* G, const float* scale,
+                  const float* shift, int apply_scale, float width, int tile, const int* wrap, int chunk_len,
+                  int* hist, int* scanned, int* scan_ws, float* tsort, int* perm, void* stream);
+int mra_bin_items(const int* scanned, int nchunks, int ntiles, int S, int prob, int* nitems_ws, int* item_off,
+                  int* scan_ws, void* items, int max_items, int* tile_start, void* stream);
+/* Kaiser-Bessel weights of every sorted sample (kernels.py:180-246 kaiser_bessel,
+ * kernels.py:295-393 weights2d/weights3d, norm '1'). */
+int mra_plan_weights(int ndim, int P, const float* tsort, int npts, const int* G, const int* wrap, float width,
+                     float beta, float* wsort, int* lbase, void* stream);
+
+/* ---- interpolation / gridding -----------------------------------------------------------
+ * mra_interp replaces ungrid (ungrid.py:34-124, Triton kernels _ungrid1d/2d/3d
+ * ungrid.py:146-495) for kernel='kaiser_bessel', norm='1', pad_mode='circular'. */
+int mra_interp(int ndim, int P, const void* grid, void* out, const float* wsort, const int* lbase,
+               const int* perm, const void* geoms, const void* items, int nitems, int ccoils, int nchunks,
+               void* stream);
+/* mra_grid_partial + mra_grid_reduce replace grid (grid.py:32-128, Triton kernels
+ * _grid1d/2d/3d grid.py:150-502, which scatter with fp32 atomics); the reduce
+ * writes every grid point, so the output needs no zeroing. */
+int mra_grid_partial(int ndim, int P, const void* ksp, const float* dcf, const float* wsort, const int* lbase,
+                     const int* perm, const void* geoms, const void* items, int nitems, int ccoils, int nchunks,
+                     void* scratch, void* stream);
+int mra_grid_reduce(int ndim, int P, void* grid, const void* scratch, const void* geoms, int prob,
+                    const int* tile_items, int ntiles, int nbatch, int accumulate, void* stream);
+/* Gradient of Re<a, interp(grid, t)> with respect to the sample coordinates t.
+ * No reference counterpart: InterpolateFn.backward returns None for locs
+ * (triton_interp/interp.py:66-74). */
+int mra_interp_grad(int ndim, int P, const void* grid, const void* a, float* grad_t, const float* tsort,
+                    const int* perm, const void* geoms, const void* items, int nitems, int max_item_samples,
+                    void* stream);
+
+/* ---- around the FFT ---------------------------------------------------------------------
+ * mra_pad_expand replaces, fused: sense_linop coil expand mps*img (linops.py:1161),
+ * apodize_img (fourier.py:398-418), the /sqrt(prod N) scale and resize zero-pad
+ * (fourier.py:456-462, utils.py:41-84) and the ifftshift of fft (fourier.py:45),
+ * or PadLast.forward (pad.py:28-31) for the Toeplitz path. */
+int mra_pad_expand(int ndim, const int* N, const int* G, const int* center, const void* x, long long nx,
+                   long long x_stride, const void* smaps, long long ns, long long s_stride, const float* apod,
+                   float scale, void* out, void* stream);
+/* mra_crop_reduce replaces, fused: the fftshift of ifft (fourier.py:58), resize crop
+ * and scaling (fourier.py:520-521), apodize_img, and the coil reduction
+ * einsum(FHWy, conj(mps)) (linops.py:1220) / PadLast.adjoint (pad.py:33-37). */
+int mra_crop_reduce(int ndim, const int* N, const int* G, const int* center, const void* grid, long long nx,
+                    const void* smaps, long long ns, long long s_stride, int conj_maps, const float* apod,
+                    float scale, int accumulate, int roll_out, void* out, long long x_stride, void* stream);
+/* Toeplitz PSF multiply (fourier.py:344, linops.py:1285-1287). */
+int mra_cmul_bcast(void* data, const void* k, long long nbatch, long long n, void* stream);
+
+/* ---- conjugate gradient -----------------------------------------------------------------
+ * Vector work of conjugate_gradient (algs.py:779-872) with FP64 inner products
+ * and device-resident step sizes. */
+int mra_cg_partials(void);
+int mra_cdot_re(const void* a, const void* b, long long n, float lam, double* part, double* sc, int slot,
+                void* stream);
+int mra_cg_init(const void* b, const void* ax, const void* x, void* r, void* p, long long n, float lam,
+                double* part, double* sc, void* stream);
+int mra_cg_step(void* x, void* r, void* p, const void* ap, long long n, float lam, double* part, double* sc,
+                void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MR_RECON_AMD_H */

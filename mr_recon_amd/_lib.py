@@ -34,16 +34,20 @@ SIGNATURES = {
     "mra_last_error": (C.c_char_p, []),
     "mra_geom_size": (_i, []),
     "mra_default_tile": (_i, [_i]),
-    "mra_samples_per_item": (_i, []),
-    "mra_geom_fill": (_i, [_vp, _ll, _ll, _ll, _ll, _ll, _i, _i, _i, _ip, _i, _ip, _f, _f, _f]),
+    "mra_grad_samples_per_block": (_i, []),
+    "mra_subgrid_points": (_ll, [_i, _i]),
+    "mra_max_coil_chunk": (_i, [_i, _i]),
+    "mra_geom_fill": (_i, [_vp, _ll, _ll, _ll, _ll, _ll, _i, _i, _i, _ip, _i, _ip, _f, _f, _f, _i, _ll, _i]),
+    "mra_plan_weights": (_i, [_i, _i, _vp, _i, _ip, _ip, _f, _f, _vp, _vp, _vp]),
     "mra_scan_workspace": (_ll, [_ll]),
     "mra_scan_excl_i32": (_i, [_vp, _ll, _vp, _vp, _vp]),
     "mra_bin_sizes": (_i, [_ll, _i, _i, _ip, _llp, _llp]),
     "mra_bin_build": (_i, [_i, _vp, _ll, _ip, _fp, _fp, _i, _f, _i, _ip, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mra_bin_items": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp]),
-    "mra_interp": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
-    "mra_grid": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
-    "mra_interp_grad": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]),
+    "mra_interp": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
+    "mra_grid_partial": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp]),
+    "mra_grid_reduce": (_i, [_i, _i, _vp, _vp, _vp, _i, _vp, _i, _i, _i, _vp]),
+    "mra_interp_grad": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
     "mra_pad_expand": (_i, [_i, _ip, _ip, _ip, _vp, _ll, _ll, _vp, _ll, _ll, _vp, _f, _vp, _vp]),
     "mra_crop_reduce": (_i, [_i, _ip, _ip, _ip, _vp, _ll, _vp, _ll, _ll, _i, _vp, _f, _i, _i, _vp, _ll, _vp]),
     "mra_cmul_bcast": (_i, [_vp, _vp, _ll, _ll, _vp]),

@@ -9,6 +9,8 @@
 //
 // Device scalar block `sc` (double[8]):
 //   sc[0] rz (= <r,r>), sc[1] pAp, sc[2] rz of the previous iteration, sc[3] beta.
+// Once the recursive residual underflows to exactly zero the step sizes are set
+// to 0 (the iterate stays put) rather than 0/0.
 #include "common.h"
 #include "capi_util.h"
 
@@ -56,7 +58,8 @@ __global__ __launch_bounds__(kRedThreads) void reduce_final_kernel(const double*
     if (mode == 0) {
       sc[slot] = t;
     } else {
-      sc[3] = t / sc[0];
+      // a residual that has reached exactly zero gives beta = 0 instead of 0/0
+      sc[3] = sc[0] > 0.0 ? t / sc[0] : 0.0;
       sc[2] = sc[0];
       sc[0] = t;
     }
@@ -69,7 +72,7 @@ __global__ __launch_bounds__(kRedThreads) void cg_xr_kernel(float2* __restrict__
                                                             const float2* __restrict__ ap, long long n, float lam,
                                                             const double* __restrict__ sc,
                                                             double* __restrict__ part) {
-  const float alpha = (float)(sc[0] / sc[1]);
+  const float alpha = sc[1] > 0.0 ? (float)(sc[0] / sc[1]) : 0.0f;  // p = 0 once converged exactly
   double s = 0.0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const float2 pv = p[i], av = ap[i];

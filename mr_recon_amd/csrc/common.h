@@ -35,6 +35,9 @@ struct Geom {
   float beta;             // Kaiser-Bessel shape
   float r;                // half width (w/2)
   float out_scale;        // multiplies every weight (e.g. 1/w^D)
+  int item_base;          // index of this problem's first work item
+  long long scratch_off;  // float2 offset of this problem's gridding partials
+  int tile_items_off;     // offset of this problem's per-tile item table
   int pad_;
 };
 

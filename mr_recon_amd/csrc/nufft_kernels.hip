@@ -1,24 +1,30 @@
 // Kaiser-Bessel interpolation (gather), gridding (scatter) and the trajectory
 // gradient of interpolation, on tile-sorted samples.
 //
-// Work decomposition (all three kernels): one 256-thread block per work item
-// = one spatial tile's slice of at most 256*K tile-sorted samples. Each thread
-// owns K samples, computes their separable KB weights once and keeps them in
-// registers for all batch entries (coils). The tile's grid neighbourhood
-// (T + P - 1 points per dim, wrapped mod G) is staged in LDS one batch entry
-// at a time; every patch read is an LDS read at a compile-time offset from the
-// sample's base.
-//
-//  interp:  LDS double-buffered; the next coil's subgrid is fetched into
-//           registers while the current one is consumed.
-//  grid:    samples accumulate into an LDS subgrid with LDS float atomics; the
-//           subgrid is then flushed to HBM with one global float atomic per
-//           non-zero (re, im) component and cleared in the same pass.
-//  grad:    like interp, with derivative weights, reduced over coils.
-//
 // Reference semantics (ungrid.py:356-495, grid.py:361-502, kernels.py:180-393):
-// out[b, p] = sum_{g in nbhd(t_p)} prod_d KB(|g_d - t_d| / r) * grid[b, g mod G]
-// with KB(u) = I0(beta sqrt(1 - u^2)), |g_d - t_d| <= r, r = w/2.
+//   out[b, p] = sum_{g in nbhd(t_p)} prod_d KB(|g_d - t_d| / r) * grid[b, g mod G]
+//   KB(u) = I0(beta sqrt(1 - u^2)),  |g_d - t_d| <= r = w/2,  g_d = ceil(t_d - r) + j.
+//
+// Decomposition. Samples are binned by the tile (T^D grid points) holding the
+// lower corner of their patch; a tile's neighbourhood ("subgrid", T + P - 1
+// points per dim, wrapped mod G) holds every grid point its samples touch.
+// A work item is one tile's slice of at most S sorted samples; a block handles
+// one (item, coil chunk) pair, so hot tiles split over samples and every tile
+// splits over coils.
+//
+//  plan_weights  once per trajectory: the D x P separable KB weights and the
+//                subgrid offset of each sorted sample.
+//  interp        the block stages its coils' subgrids in LDS, then each thread
+//                walks samples: 25 weight products, P^D LDS reads at
+//                compile-time offsets, one output per coil.
+//  grid_partial  the block accumulates its samples into LDS subgrids with LDS
+//                float atomics and stores each subgrid to a scratch slot with
+//                plain (coalesced) stores -- no global atomics.
+//  grid_reduce   one block per (output tile, coil) sums, in a fixed order, the
+//                scratch subgrids of every item whose subgrid covers the tile
+//                and writes the grid once. Device-scope float atomics run at a
+//                few G/s on MI355X, so gridding never uses them.
+//  interp_grad   derivative weights, reduced over coils per sample.
 #include "common.h"
 #include "capi_util.h"
 
@@ -31,7 +37,16 @@ struct TileShape {
   static constexpr int SUBD1 = ipow(SUB, D - 1);   // stride of dim 0 in the subgrid
   static constexpr int PR = ipow(P, D - 1);        // patch points in dims 1..D-1
   static constexpr int NQ = (SUBN + kThreads - 1) / kThreads;
+  static constexpr int TN = ipow(T, D);             // tile points
 };
+
+// Tile edge per dimensionality (grid points).
+template <int D> struct TileCfg;
+template <> struct TileCfg<1> { static constexpr int T = 256; };
+template <> struct TileCfg<2> { static constexpr int T = 32; };
+template <> struct TileCfg<3> { static constexpr int T = 8; };
+
+constexpr int kLdsBudget = 64 * 1024;  // bytes of LDS subgrids per block
 
 // Offset in the subgrid of patch point q of the trailing D-1 dims.
 template <int D, int P, int T>
@@ -43,18 +58,18 @@ __device__ __forceinline__ constexpr int rest_off(int q) {
 }
 
 template <int D, int T>
-__device__ __forceinline__ void tile_origin(int tile, const Geom& g, int (&org)[D]) {
+__device__ __forceinline__ void tile_coords(int tile, const Geom& g, int (&tc)[D]) {
   int rem = tile;
 #pragma unroll
   for (int d = D - 1; d >= 0; --d) {
-    org[d] = (rem % g.ntile[d]) * T;
+    tc[d] = rem % g.ntile[d];
     rem /= g.ntile[d];
   }
 }
 
 // Global (wrapped) linear offsets of the subgrid points this thread moves.
 template <int D, int P, int T>
-__device__ __forceinline__ void subgrid_offsets(const Geom& g, const int (&org)[D],
+__device__ __forceinline__ void subgrid_offsets(const Geom& g, const int (&tc)[D],
                                                 int (&goff)[TileShape<D, P, T>::NQ]) {
   using TS = TileShape<D, P, T>;
 #pragma unroll
@@ -69,7 +84,7 @@ __device__ __forceinline__ void subgrid_offsets(const Geom& g, const int (&org)[
       }
       int lin = 0;
 #pragma unroll
-      for (int d = 0; d < D; ++d) lin = lin * g.G[d] + (org[d] + c[d]) % g.G[d];
+      for (int d = 0; d < D; ++d) lin = lin * g.G[d] + (tc[d] * T + c[d]) % g.G[d];
       goff[q] = lin;
     } else {
       goff[q] = -1;
@@ -77,114 +92,108 @@ __device__ __forceinline__ void subgrid_offsets(const Geom& g, const int (&org)[
   }
 }
 
-// Weights and LDS base of one sample. Returns the LDS base offset.
-template <int D, int P, int T>
-__device__ __forceinline__ int sample_weights(const float* __restrict__ tsort, long long s, const Geom& g,
-                                              const int (&org)[D], float (&w)[D][P]) {
-  constexpr int SUB = T + P - 1;
-  int base = 0;
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    const float t = tsort[s * D + d];
-    const int lower = kb_weights<P>(t, g.r, g.beta, d == 0 ? g.out_scale : 1.0f, w[d]);
-    const int lw = pmod(lower + g.wrap[d], g.G[d]) - org[d];
-    base = base * SUB + lw;
-  }
-  return base;
-}
-
 template <int D, int P>
-__device__ __forceinline__ void rest_products(const float (&w)[D][P], float (&wr)[ipow(P, D - 1)]) {
+__device__ __forceinline__ void load_weights(const float* __restrict__ wsort, long long s, float scale,
+                                             float (&wx)[P], float (&wr)[ipow(P, D - 1)]) {
+  const float* w = wsort + s * (D * P);
+  float wd[D][P];
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+#pragma unroll
+    for (int j = 0; j < P; ++j) wd[d][j] = w[d * P + j];
+#pragma unroll
+  for (int i = 0; i < P; ++i) wx[i] = wd[0][i] * scale;
   if constexpr (D == 3) {
 #pragma unroll
     for (int j = 0; j < P; ++j)
 #pragma unroll
-      for (int l = 0; l < P; ++l) wr[j * P + l] = w[1][j] * w[2][l];
+      for (int l = 0; l < P; ++l) wr[j * P + l] = wd[1][j] * wd[2][l];
   } else if constexpr (D == 2) {
 #pragma unroll
-    for (int l = 0; l < P; ++l) wr[l] = w[1][l];
+    for (int l = 0; l < P; ++l) wr[l] = wd[1][l];
   } else {
     wr[0] = 1.0f;
   }
 }
 
 // --------------------------------------------------------------------------------------------
-template <int D, int P, int T, int K>
+// Per sorted sample: KB weights along each dim (unscaled) and the subgrid base offset.
+template <int D, int P, int T>
+__global__ __launch_bounds__(256) void plan_weights_kernel(const float* __restrict__ tsort, int npts, Geom g,
+                                                           float* __restrict__ wsort, int* __restrict__ lbase) {
+  constexpr int SUB = T + P - 1;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= npts) return;
+  float w[P];
+  int base = 0;
+  bool ok = true;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const float t = tsort[(long long)s * D + d];
+    ok = ok && isfinite(t);
+    const int lower = ok ? kb_weights<P>(t, g.r, g.beta, 1.0f, w) : 0;
+    const int lw = ok ? pmod(lower + g.wrap[d], g.G[d]) % T : 0;
+    base = base * SUB + lw;
+#pragma unroll
+    for (int j = 0; j < P; ++j) wsort[(long long)s * D * P + d * P + j] = ok ? w[j] : 0.0f;
+  }
+  lbase[s] = ok ? base : 0;
+}
+
+// --------------------------------------------------------------------------------------------
+template <int D, int P, int T>
 __global__ __launch_bounds__(kThreads) void interp_kernel(const float2* __restrict__ grid,
                                                           float2* __restrict__ out,
-                                                          const float* __restrict__ tsort,
+                                                          const float* __restrict__ wsort,
+                                                          const int* __restrict__ lbase,
                                                           const int* __restrict__ perm,
                                                           const Geom* __restrict__ geoms,
-                                                          const Item* __restrict__ items, int nitems) {
+                                                          const Item* __restrict__ items, int nitems, int ccoils,
+                                                          int nchunks) {
   using TS = TileShape<D, P, T>;
   extern __shared__ float2 lds2[];
-  const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
-  if (bid >= nitems) return;
-  const Item it = items[bid];
+  const int L = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int iidx = L / nchunks, cc = L - iidx * nchunks;
+  if (iidx >= nitems) return;
+  const Item it = items[iidx];
   if (it.w <= 0) return;
   const Geom g = geoms[it.x];
+  const int c0 = cc * ccoils;
+  if (c0 >= g.nbatch) return;
+  const int nc = min(ccoils, g.nbatch - c0);
   const int tid = threadIdx.x;
-  float2* odst = out + g.ksp_off;
+  float2* odst = out + g.ksp_off + (long long)c0 * g.ksp_stride;
 
   if (it.y < 0) {  // samples with non-finite coordinates: defined as 0
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int s = it.z + tid + k * kThreads;
-      if (s < it.z + it.w) {
-        const int o = perm[g.samp_off + s];
-        for (int b = 0; b < g.nbatch; ++b) odst[(long long)b * g.ksp_stride + o] = make_float2(0.f, 0.f);
-      }
+    for (int s = it.z + tid; s < it.z + it.w; s += kThreads) {
+      const int o = perm[g.samp_off + s];
+      for (int cl = 0; cl < nc; ++cl) odst[(long long)cl * g.ksp_stride + o] = make_float2(0.f, 0.f);
     }
     return;
   }
 
-  int org[D];
-  tile_origin<D, T>(it.y, g, org);
-  int goff[TS::NQ];
-  subgrid_offsets<D, P, T>(g, org, goff);
-
-  float wx[K][P];
-  float wr[K][TS::PR];
-  int base[K], oidx[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int s = it.z + tid + k * kThreads;
-    if (s < it.z + it.w) {
-      float w[D][P];
-      base[k] = sample_weights<D, P, T>(tsort, g.samp_off + s, g, org, w);
-#pragma unroll
-      for (int i = 0; i < P; ++i) wx[k][i] = w[0][i];
-      rest_products<D, P>(w, wr[k]);
-      oidx[k] = perm[g.samp_off + s];
-    } else {
-#pragma unroll
-      for (int i = 0; i < P; ++i) wx[k][i] = 0.f;
-#pragma unroll
-      for (int q = 0; q < TS::PR; ++q) wr[k][q] = 0.f;
-      base[k] = 0;
-      oidx[k] = -1;
-    }
-  }
-
-  const float2* gsrc = grid + g.grid_off;
-  float2 pre[TS::NQ];
-#pragma unroll
-  for (int q = 0; q < TS::NQ; ++q)
-    if (goff[q] >= 0) lds2[tid + q * kThreads] = gsrc[goff[q]];
-  __syncthreads();
-
-  for (int b = 0; b < g.nbatch; ++b) {
-    const int cur = (b & 1) * TS::SUBN;
-    const bool more = b + 1 < g.nbatch;
-    if (more) {
-      const float2* src = gsrc + (long long)(b + 1) * g.grid_stride;
+  int tc[D];
+  tile_coords<D, T>(it.y, g, tc);
+  {
+    int goff[TS::NQ];
+    subgrid_offsets<D, P, T>(g, tc, goff);
+    const float2* gsrc = grid + g.grid_off + (long long)c0 * g.grid_stride;
+    for (int cl = 0; cl < nc; ++cl) {
 #pragma unroll
       for (int q = 0; q < TS::NQ; ++q)
-        if (goff[q] >= 0) pre[q] = src[goff[q]];
+        if (goff[q] >= 0) lds2[cl * TS::SUBN + tid + q * kThreads] = gsrc[(long long)cl * g.grid_stride + goff[q]];
     }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const float2* sb = lds2 + cur + base[k];
+  }
+  __syncthreads();
+
+  for (int s = it.z + tid; s < it.z + it.w; s += kThreads) {
+    const long long gs = g.samp_off + s;
+    float wx[P], wr[TS::PR];
+    load_weights<D, P>(wsort, gs, g.out_scale, wx, wr);
+    const int base = lbase[gs];
+    const int o = perm[gs];
+    for (int cl = 0; cl < nc; ++cl) {
+      const float2* sb = lds2 + cl * TS::SUBN + base;
       float ax = 0.f, ay = 0.f;
 #pragma unroll
       for (int i = 0; i < P; ++i) {
@@ -192,111 +201,190 @@ __global__ __launch_bounds__(kThreads) void interp_kernel(const float2* __restri
 #pragma unroll
         for (int q = 0; q < TS::PR; ++q) {
           const float2 v = sb[i * TS::SUBD1 + rest_off<D, P, T>(q)];
-          bx += wr[k][q] * v.x;
-          by += wr[k][q] * v.y;
+          bx += wr[q] * v.x;
+          by += wr[q] * v.y;
         }
-        ax += wx[k][i] * bx;
-        ay += wx[k][i] * by;
+        ax += wx[i] * bx;
+        ay += wx[i] * by;
       }
-      if (oidx[k] >= 0) odst[(long long)b * g.ksp_stride + oidx[k]] = make_float2(ax, ay);
+      odst[(long long)cl * g.ksp_stride + o] = make_float2(ax, ay);
     }
-    if (more) {
-      const int nxt = TS::SUBN - cur;
-#pragma unroll
-      for (int q = 0; q < TS::NQ; ++q)
-        if (goff[q] >= 0) lds2[nxt + tid + q * kThreads] = pre[q];
-    }
-    __syncthreads();
   }
 }
 
 // --------------------------------------------------------------------------------------------
-template <int D, int P, int T, int K>
-__global__ __launch_bounds__(kThreads) void grid_kernel(float2* __restrict__ grid, const float2* __restrict__ ksp,
-                                                        const float* __restrict__ dcf,
-                                                        const float* __restrict__ tsort,
-                                                        const int* __restrict__ perm,
-                                                        const Geom* __restrict__ geoms,
-                                                        const Item* __restrict__ items, int nitems) {
+// Scratch slot of (item, batch entry): scratch + scratch_off + ((item - item_base) * nbatch + b) * SUBN
+template <int D, int P, int T>
+__global__ __launch_bounds__(kThreads) void grid_partial_kernel(const float2* __restrict__ ksp,
+                                                                const float* __restrict__ dcf,
+                                                                const float* __restrict__ wsort,
+                                                                const int* __restrict__ lbase,
+                                                                const int* __restrict__ perm,
+                                                                const Geom* __restrict__ geoms,
+                                                                const Item* __restrict__ items, int nitems,
+                                                                int ccoils, int nchunks,
+                                                                float2* __restrict__ scratch) {
   using TS = TileShape<D, P, T>;
   extern __shared__ float ldsf[];
-  const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
-  if (bid >= nitems) return;
-  const Item it = items[bid];
+  const int L = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int iidx = L / nchunks, cc = L - iidx * nchunks;
+  if (iidx >= nitems) return;
+  const Item it = items[iidx];
   if (it.w <= 0 || it.y < 0) return;  // non-finite samples contribute nothing
   const Geom g = geoms[it.x];
+  const int c0 = cc * ccoils;
+  if (c0 >= g.nbatch) return;
+  const int nc = min(ccoils, g.nbatch - c0);
   const int tid = threadIdx.x;
 
-  int org[D];
-  tile_origin<D, T>(it.y, g, org);
-  int goff[TS::NQ];
-  subgrid_offsets<D, P, T>(g, org, goff);
-
-  float wx[K][P];
-  float wr[K][TS::PR];
-  float dk[K];
-  int base[K], oidx[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int s = it.z + tid + k * kThreads;
-    if (s < it.z + it.w) {
-      float w[D][P];
-      base[k] = sample_weights<D, P, T>(tsort, g.samp_off + s, g, org, w);
-#pragma unroll
-      for (int i = 0; i < P; ++i) wx[k][i] = w[0][i];
-      rest_products<D, P>(w, wr[k]);
-      oidx[k] = perm[g.samp_off + s];
-      dk[k] = dcf ? dcf[g.samp_off + oidx[k]] : 1.0f;
-    } else {
-      base[k] = 0;
-      oidx[k] = -1;
-      dk[k] = 0.f;
-    }
-  }
-
-  for (int e = tid; e < 2 * TS::SUBN; e += kThreads) ldsf[e] = 0.f;
+  for (int e = tid; e < 2 * nc * TS::SUBN; e += kThreads) ldsf[e] = 0.f;
   __syncthreads();
 
-  const float2* ksrc = ksp + g.ksp_off;
-  float2* gdst = grid + g.grid_off;
-  for (int b = 0; b < g.nbatch; ++b) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      if (oidx[k] < 0) continue;
-      float2 v = ksrc[(long long)b * g.ksp_stride + oidx[k]];
-      v.x *= dk[k];
-      v.y *= dk[k];
-      float* sb = ldsf + 2 * base[k];
+  const float2* ksrc = ksp + g.ksp_off + (long long)c0 * g.ksp_stride;
+  for (int s = it.z + tid; s < it.z + it.w; s += kThreads) {
+    const long long gs = g.samp_off + s;
+    float wx[P], wr[TS::PR];
+    load_weights<D, P>(wsort, gs, g.out_scale, wx, wr);
+    const int base = lbase[gs];
+    const int o = perm[gs];
+    const float dk = dcf ? dcf[g.samp_off + o] : 1.0f;
+    for (int cl = 0; cl < nc; ++cl) {
+      float2 v = ksrc[(long long)cl * g.ksp_stride + o];
+      v.x *= dk;
+      v.y *= dk;
+      float* sb = ldsf + 2 * (cl * TS::SUBN + base);
 #pragma unroll
       for (int i = 0; i < P; ++i) {
-        const float vx = wx[k][i] * v.x, vy = wx[k][i] * v.y;
+        const float vx = wx[i] * v.x, vy = wx[i] * v.y;
 #pragma unroll
         for (int q = 0; q < TS::PR; ++q) {
-          const int o = 2 * (i * TS::SUBD1 + rest_off<D, P, T>(q));
-          __hip_atomic_fetch_add(sb + o, wr[k][q] * vx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          __hip_atomic_fetch_add(sb + o + 1, wr[k][q] * vy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          const int off = 2 * (i * TS::SUBD1 + rest_off<D, P, T>(q));
+          __hip_atomic_fetch_add(sb + off, wr[q] * vx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(sb + off + 1, wr[q] * vy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
       }
     }
-    __syncthreads();
-    float2* dst = gdst + (long long)b * g.grid_stride;
+  }
+  __syncthreads();
+  float2* dst = scratch + g.scratch_off + ((long long)(iidx - g.item_base) * g.nbatch + c0) * TS::SUBN;
+  const float2* src = reinterpret_cast<const float2*>(ldsf);
+  for (int e = tid; e < nc * TS::SUBN; e += kThreads) dst[e] = src[e];
+}
+
+// One block per (output tile, batch entry): sums the covering subgrids in a fixed order.
+template <int D, int P, int T>
+__global__ __launch_bounds__(kThreads) void grid_reduce_kernel(float2* __restrict__ grid,
+                                                               const float2* __restrict__ scratch,
+                                                               const Geom* __restrict__ geoms, int prob,
+                                                               const int* __restrict__ tile_items, int accumulate) {
+  using TS = TileShape<D, P, T>;
+  const Geom g = geoms[prob];
+  const int tile = blockIdx.x;
+  const int b = blockIdx.y;
+  int tc[D];
+  tile_coords<D, T>(tile, g, tc);
+  // Candidate source tiles per dim: t, t-1, t-2 (mod ntile), those whose subgrid
+  // interval [org', org' + SUB) mod G can reach this tile.
+  int cand[D][3], ncand[D];
 #pragma unroll
-    for (int q = 0; q < TS::NQ; ++q) {
-      const int e = tid + q * kThreads;
-      if (goff[q] >= 0) {
-        const float2 v = reinterpret_cast<float2*>(ldsf)[e];
-        reinterpret_cast<float2*>(ldsf)[e] = make_float2(0.f, 0.f);
-        if (v.x != 0.f) unsafeAtomicAdd(&dst[goff[q]].x, v.x);
-        if (v.y != 0.f) unsafeAtomicAdd(&dst[goff[q]].y, v.y);
-      }
+  for (int d = 0; d < D; ++d) {
+    ncand[d] = 0;
+    const int kmax = min(3, g.ntile[d]);
+    for (int k = 0; k < kmax; ++k) {
+      const int tp = pmod(tc[d] - k, g.ntile[d]);
+      // [org', org' + SUB) mod G meets [org, org + T) iff the tile starts inside
+      // the subgrid (dist < SUB) or wraps past org' + G (dist + T - 1 >= G).
+      const int dist = pmod(tc[d] * T - tp * T, g.G[d]);
+      if (dist < TS::SUB || dist + T - 1 >= g.G[d]) cand[d][ncand[d]++] = tp;
     }
-    __syncthreads();
+  }
+  const int* titems = tile_items + g.tile_items_off;
+  const float2* sbase = scratch + g.scratch_off;
+  float2* gdst = grid + g.grid_off + (long long)b * g.grid_stride;
+  for (int p = threadIdx.x; p < TS::TN; p += kThreads) {
+    int gc[D];
+    {
+      int rem = p;
+      bool inside = true;
+#pragma unroll
+      for (int d = D - 1; d >= 0; --d) {
+        gc[d] = tc[d] * T + rem % T;
+        rem /= T;
+        inside = inside && gc[d] < g.G[d];
+      }
+      if (!inside) continue;
+    }
+    float ax = 0.f, ay = 0.f;
+    // iterate over candidate tile combinations (row-major over dims)
+    int k[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) k[d] = 0;
+    while (true) {
+      bool ok = true;
+      int stile = 0;
+      int loc0[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const int tp = cand[d][k[d]];
+        loc0[d] = pmod(gc[d] - tp * T, g.G[d]);
+        ok = ok && loc0[d] < TS::SUB;
+        stile = stile * g.ntile[d] + tp;
+      }
+      if (ok) {
+        const int i0 = titems[stile], i1 = titems[stile + 1];
+        for (int it = i0; it < i1; ++it) {
+          const float2* slot = sbase + ((long long)(it - g.item_base) * g.nbatch + b) * TS::SUBN;
+          // a subgrid can wrap onto a grid point more than once when G < SUB
+          if constexpr (D == 1) {
+            for (int e0 = loc0[0]; e0 < TS::SUB; e0 += g.G[0]) {
+              const float2 v = slot[e0];
+              ax += v.x; ay += v.y;
+            }
+          } else if constexpr (D == 2) {
+            for (int e0 = loc0[0]; e0 < TS::SUB; e0 += g.G[0])
+              for (int e1 = loc0[1]; e1 < TS::SUB; e1 += g.G[1]) {
+                const float2 v = slot[e0 * TS::SUB + e1];
+                ax += v.x; ay += v.y;
+              }
+          } else {
+            for (int e0 = loc0[0]; e0 < TS::SUB; e0 += g.G[0])
+              for (int e1 = loc0[1]; e1 < TS::SUB; e1 += g.G[1])
+                for (int e2 = loc0[2]; e2 < TS::SUB; e2 += g.G[2]) {
+                  const float2 v = slot[(e0 * TS::SUB + e1) * TS::SUB + e2];
+                  ax += v.x; ay += v.y;
+                }
+          }
+        }
+      }
+      // next combination
+      int d = D - 1;
+      while (d >= 0) {
+        if (++k[d] < ncand[d]) break;
+        k[d] = 0;
+        --d;
+      }
+      if (d < 0) break;
+    }
+    long long lin = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) lin = lin * g.G[d] + gc[d];
+    if (accumulate) {
+      float2 o = gdst[lin];
+      o.x += ax;
+      o.y += ay;
+      gdst[lin] = o;
+    } else {
+      gdst[lin] = make_float2(ax, ay);
+    }
   }
 }
 
 // --------------------------------------------------------------------------------------------
 // Trajectory gradient of interpolation, reduced over batch entries:
 //   gt[o, d] = sum_b Re( conj(a[b, o]) * sum_p dW_p/dt_d * grid[b, g_p] )
+// One block per (item, 256*K-sample sub-slice); all coils in the block, the
+// coil subgrid double-buffered in LDS; weights and derivative weights are
+// computed on the fly from the sorted coordinates.
 template <int D, int P, int T, int K>
 __global__ __launch_bounds__(kThreads) void interp_grad_kernel(const float2* __restrict__ grid,
                                                                const float2* __restrict__ a,
@@ -304,20 +392,24 @@ __global__ __launch_bounds__(kThreads) void interp_grad_kernel(const float2* __r
                                                                const float* __restrict__ tsort,
                                                                const int* __restrict__ perm,
                                                                const Geom* __restrict__ geoms,
-                                                               const Item* __restrict__ items, int nitems) {
+                                                               const Item* __restrict__ items, int nitems,
+                                                               int nsub) {
   using TS = TileShape<D, P, T>;
   extern __shared__ float2 lds2[];
-  const int bid = xcd_swizzle(blockIdx.x, gridDim.x);
-  if (bid >= nitems) return;
-  const Item it = items[bid];
-  if (it.w <= 0) return;
+  const int L = xcd_swizzle(blockIdx.x, gridDim.x);
+  const int iidx = L / nsub, sub = L - iidx * nsub;
+  if (iidx >= nitems) return;
+  const Item it = items[iidx];
+  const int s0 = it.z + sub * kThreads * K;
+  const int s1 = min(it.z + it.w, s0 + kThreads * K);
+  if (s0 >= s1) return;
   const Geom g = geoms[it.x];
   const int tid = threadIdx.x;
   if (it.y < 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int s = it.z + tid + k * kThreads;
-      if (s < it.z + it.w) {
+      const int s = s0 + tid + k * kThreads;
+      if (s < s1) {
         const int o = perm[g.samp_off + s];
 #pragma unroll
         for (int d = 0; d < D; ++d) gt[(g.samp_off + o) * D + d] = 0.f;
@@ -326,18 +418,18 @@ __global__ __launch_bounds__(kThreads) void interp_grad_kernel(const float2* __r
     return;
   }
 
-  int org[D];
-  tile_origin<D, T>(it.y, g, org);
+  int tc[D];
+  tile_coords<D, T>(it.y, g, tc);
   int goff[TS::NQ];
-  subgrid_offsets<D, P, T>(g, org, goff);
+  subgrid_offsets<D, P, T>(g, tc, goff);
 
+  constexpr int SUB = T + P - 1;
   float w[K][D][P], dw[K][D][P];
   int base[K], oidx[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const int s = it.z + tid + k * kThreads;
-    if (s < it.z + it.w) {
-      constexpr int SUB = T + P - 1;
+    const int s = s0 + tid + k * kThreads;
+    if (s < s1) {
       int bs = 0;
 #pragma unroll
       for (int d = 0; d < D; ++d) {
@@ -345,7 +437,7 @@ __global__ __launch_bounds__(kThreads) void interp_grad_kernel(const float2* __r
         const float sc = d == 0 ? g.out_scale : 1.0f;
         const int lower = kb_weights<P>(t, g.r, g.beta, sc, w[k][d]);
         kb_dweights<P>(t, g.r, g.beta, sc, lower, dw[k][d]);
-        bs = bs * SUB + (pmod(lower + g.wrap[d], g.G[d]) - org[d]);
+        bs = bs * SUB + pmod(lower + g.wrap[d], g.G[d]) % T;
       }
       base[k] = bs;
       oidx[k] = perm[g.samp_off + s];
@@ -373,7 +465,6 @@ __global__ __launch_bounds__(kThreads) void interp_grad_kernel(const float2* __r
     if (goff[q] >= 0) lds2[tid + q * kThreads] = gsrc[goff[q]];
   __syncthreads();
 
-  constexpr int SUB = T + P - 1;
   for (int b = 0; b < g.nbatch; ++b) {
     const int cur = (b & 1) * TS::SUBN;
     const bool more = b + 1 < g.nbatch;
@@ -450,79 +541,115 @@ __global__ __launch_bounds__(kThreads) void interp_grad_kernel(const float2* __r
       for (int d = 0; d < D; ++d) gt[(g.samp_off + oidx[k]) * D + d] = acc[k][d];
 }
 
-// Tile edge per dimensionality (grid points) and samples per thread.
-template <int D> struct TileCfg;
-template <> struct TileCfg<1> { static constexpr int T = 256; };
-template <> struct TileCfg<2> { static constexpr int T = 32; };
-template <> struct TileCfg<3> { static constexpr int T = 8; };
-constexpr int kK = 2;
+constexpr int kGradK = 2;
 
-enum class Op { Interp, Grid, Grad };
+// --------------------------------------------------------------------------------------------
+enum class Op { Weights, Interp, GridPartial, GridReduce, Grad };
 
 struct LaunchArgs {
-  const void* src;     // grid (interp, grad) or ksp (grid)
-  void* dst;           // ksp (interp), grid (grid), gt (grad)
-  const void* aux;     // dcf (grid) or a (grad)
+  Op op;
+  const void* src;      // grid (interp, grad, reduce: scratch) / ksp (grid_partial) / tsort (weights)
+  void* dst;            // out ksp / scratch / grid / gt / wsort
+  const void* aux;      // dcf (grid_partial) / a (grad)
+  const float* wsort;
+  const int* lbase;     // lbase (also written by Weights)
   const float* tsort;
   const int* perm;
-  const Geom* geoms;
+  const Geom* geoms;    // device table (Weights: host copy in `geom`)
+  Geom geom;
   const Item* items;
   int nitems;
+  int ccoils, nchunks;  // coils per block, coil chunks
+  int prob, ntiles, nbatch, accumulate;
+  const int* tile_items;
+  int npts;
   hipStream_t stream;
 };
 
+static unsigned pad8(long long n) { return (unsigned)((n + 7) / 8 * 8); }
+
 template <int D, int P>
-static int launch_op(Op op, const LaunchArgs& a) {
+static int launch_op(const LaunchArgs& a) {
   constexpr int T = TileCfg<D>::T;
   using TS = TileShape<D, P, T>;
-  if (a.nitems <= 0) return 0;
-  const int nblk = (a.nitems + 7) / 8 * 8;
-  switch (op) {
-    case Op::Interp:
-      interp_kernel<D, P, T, kK><<<nblk, kThreads, 2 * TS::SUBN * sizeof(float2), a.stream>>>(
-          (const float2*)a.src, (float2*)a.dst, a.tsort, a.perm, a.geoms, a.items, a.nitems);
+  switch (a.op) {
+    case Op::Weights:
+      if (a.npts > 0)
+        plan_weights_kernel<D, P, T><<<(a.npts + 255) / 256, 256, 0, a.stream>>>(
+            (const float*)a.src, a.npts, a.geom, (float*)a.dst, (int*)a.lbase);
       break;
-    case Op::Grid:
-      grid_kernel<D, P, T, kK><<<nblk, kThreads, TS::SUBN * sizeof(float2), a.stream>>>(
-          (float2*)a.dst, (const float2*)a.src, (const float*)a.aux, a.tsort, a.perm, a.geoms, a.items,
-          a.nitems);
+    case Op::Interp: {
+      if (a.nitems <= 0) break;
+      const size_t sh = (size_t)a.ccoils * TS::SUBN * sizeof(float2);
+      interp_kernel<D, P, T><<<pad8((long long)a.nitems * a.nchunks), kThreads, sh, a.stream>>>(
+          (const float2*)a.src, (float2*)a.dst, a.wsort, a.lbase, a.perm, a.geoms, a.items, a.nitems, a.ccoils,
+          a.nchunks);
       break;
-    case Op::Grad:
-      interp_grad_kernel<D, P, T, kK><<<nblk, kThreads, 2 * TS::SUBN * sizeof(float2), a.stream>>>(
-          (const float2*)a.src, (const float2*)a.aux, (float*)a.dst, a.tsort, a.perm, a.geoms, a.items,
-          a.nitems);
+    }
+    case Op::GridPartial: {
+      if (a.nitems <= 0) break;
+      const size_t sh = (size_t)a.ccoils * TS::SUBN * sizeof(float2);
+      grid_partial_kernel<D, P, T><<<pad8((long long)a.nitems * a.nchunks), kThreads, sh, a.stream>>>(
+          (const float2*)a.src, (const float*)a.aux, a.wsort, a.lbase, a.perm, a.geoms, a.items, a.nitems,
+          a.ccoils, a.nchunks, (float2*)a.dst);
       break;
+    }
+    case Op::GridReduce: {
+      if (a.ntiles <= 0 || a.nbatch <= 0) break;
+      if (a.nbatch > 65535) {
+        set_last_error("grid_reduce: batch larger than 65535");
+        return 1;
+      }
+      dim3 grd((unsigned)a.ntiles, (unsigned)a.nbatch);
+      grid_reduce_kernel<D, P, T><<<grd, kThreads, 0, a.stream>>>((float2*)a.dst, (const float2*)a.src, a.geoms,
+                                                                  a.prob, a.tile_items, a.accumulate);
+      break;
+    }
+    case Op::Grad: {
+      if (a.nitems <= 0) break;
+      const int nsub = a.nchunks;  // sub-slices of kThreads*kGradK samples per item
+      interp_grad_kernel<D, P, T, kGradK><<<pad8((long long)a.nitems * nsub), kThreads,
+                                            2 * TS::SUBN * sizeof(float2), a.stream>>>(
+          (const float2*)a.src, (const float2*)a.aux, (float*)a.dst, a.tsort, a.perm, a.geoms, a.items, a.nitems,
+          nsub);
+      break;
+    }
   }
   MRA_CHECK_LAUNCH("nufft kernel");
   return 0;
 }
 
 template <int D>
-static int dispatch_p(int P, Op op, const LaunchArgs& a) {
+static int dispatch_p(int P, const LaunchArgs& a) {
   switch (P) {
-    case 2: return launch_op<D, 2>(op, a);
-    case 3: return launch_op<D, 3>(op, a);
-    case 4: return launch_op<D, 4>(op, a);
-    case 5: return launch_op<D, 5>(op, a);
-    case 6: return launch_op<D, 6>(op, a);
-    case 7: return launch_op<D, 7>(op, a);
-    case 8: return launch_op<D, 8>(op, a);
-    case 9: return launch_op<D, 9>(op, a);
+    case 2: return launch_op<D, 2>(a);
+    case 3: return launch_op<D, 3>(a);
+    case 4: return launch_op<D, 4>(a);
+    case 5: return launch_op<D, 5>(a);
+    case 6: return launch_op<D, 6>(a);
+    case 7: return launch_op<D, 7>(a);
+    case 8: return launch_op<D, 8>(a);
+    case 9: return launch_op<D, 9>(a);
     default:
       set_last_error("kernel width unsupported: floor(width)+1 must be in 2..9");
       return 1;
   }
 }
 
-static int dispatch(int ndim, int P, Op op, const LaunchArgs& a) {
+static int dispatch(int ndim, int P, const LaunchArgs& a) {
   switch (ndim) {
-    case 1: return dispatch_p<1>(P, op, a);
-    case 2: return dispatch_p<2>(P, op, a);
-    case 3: return dispatch_p<3>(P, op, a);
+    case 1: return dispatch_p<1>(P, a);
+    case 2: return dispatch_p<2>(P, a);
+    case 3: return dispatch_p<3>(P, a);
     default:
       set_last_error("ndim must be 1, 2 or 3");
       return 1;
   }
+}
+
+static int subgrid_points(int ndim, int P) {
+  const int T = ndim == 1 ? TileCfg<1>::T : ndim == 2 ? TileCfg<2>::T : TileCfg<3>::T;
+  return ipow(T + P - 1, ndim);
 }
 
 }  // namespace mra
@@ -542,14 +669,26 @@ int mra_default_tile(int ndim) {
   }
 }
 
-int mra_samples_per_item() { return kThreads * kK; }
+int mra_grad_samples_per_block() { return kThreads * kGradK; }
+
+// Subgrid points per tile (scratch slot size of gridding) for (ndim, P).
+long long mra_subgrid_points(int ndim, int P) { return subgrid_points(ndim, P); }
+
+// Largest coil chunk whose subgrids fit the per-block LDS budget.
+int mra_max_coil_chunk(int ndim, int P) {
+  const int n = subgrid_points(ndim, P);
+  int c = kLdsBudget / (n * (int)sizeof(float2));
+  return c < 1 ? 1 : c;
+}
 
 // Fill one Geom record (host memory) for a problem.
 int mra_geom_fill(void* dst, long long grid_off, long long grid_stride, long long ksp_off, long long ksp_stride,
                   long long samp_off, int nbatch, int npts, int ndim, const int* G, int tile, const int* wrap,
-                  float beta, float width, float out_scale) {
+                  float beta, float width, float out_scale, int item_base, long long scratch_off,
+                  int tile_items_off) {
   MRA_TRY_BEGIN
   if (ndim < 1 || ndim > kMaxDim) MRA_FAIL("geom: ndim must be 1..3");
+  if (tile != mra_default_tile(ndim)) MRA_FAIL("geom: tile size does not match the compiled kernels");
   Geom g{};
   g.grid_off = grid_off;
   g.grid_stride = grid_stride;
@@ -569,28 +708,115 @@ int mra_geom_fill(void* dst, long long grid_off, long long grid_stride, long lon
   g.beta = beta;
   g.r = 0.5f * width;
   g.out_scale = out_scale;
+  g.item_base = item_base;
+  g.scratch_off = scratch_off;
+  g.tile_items_off = tile_items_off;
   *(Geom*)dst = g;
   MRA_TRY_END
 }
 
-int mra_interp(int ndim, int P, const void* grid, void* out, const float* tsort, const int* perm,
-               const void* geoms, const void* items, int nitems, void* stream) {
-  LaunchArgs a{grid, out, nullptr, tsort, perm, (const Geom*)geoms, (const Item*)items, nitems,
-               (hipStream_t)stream};
-  return dispatch(ndim, P, Op::Interp, a);
+// Per sorted sample KB weights (npts x ndim x P floats) and subgrid offsets.
+int mra_plan_weights(int ndim, int P, const float* tsort, int npts, const int* G, const int* wrap, float width,
+                     float beta, float* wsort, int* lbase, void* stream) {
+  MRA_TRY_BEGIN
+  if (ndim < 1 || ndim > 3) MRA_FAIL("plan_weights: ndim must be 1..3");
+  LaunchArgs a{};
+  a.op = Op::Weights;
+  a.src = tsort;
+  a.dst = wsort;
+  a.lbase = lbase;
+  a.npts = npts;
+  for (int d = 0; d < kMaxDim; ++d) {
+    a.geom.G[d] = d < ndim ? G[d] : 1;
+    a.geom.wrap[d] = (d < ndim && wrap) ? wrap[d] : 0;
+  }
+  a.geom.r = 0.5f * width;
+  a.geom.beta = beta;
+  a.stream = (hipStream_t)stream;
+  return dispatch(ndim, P, a);
+  MRA_TRY_END
 }
 
-int mra_grid(int ndim, int P, void* grid, const void* ksp, const float* dcf, const float* tsort, const int* perm,
-             const void* geoms, const void* items, int nitems, void* stream) {
-  LaunchArgs a{ksp, grid, dcf, tsort, perm, (const Geom*)geoms, (const Item*)items, nitems, (hipStream_t)stream};
-  return dispatch(ndim, P, Op::Grid, a);
+int mra_interp(int ndim, int P, const void* grid, void* out, const float* wsort, const int* lbase, const int* perm,
+               const void* geoms, const void* items, int nitems, int ccoils, int nchunks, void* stream) {
+  MRA_TRY_BEGIN
+  if (ccoils < 1 || ccoils > mra_max_coil_chunk(ndim, P)) MRA_FAIL("interp: coil chunk exceeds the LDS budget");
+  LaunchArgs a{};
+  a.op = Op::Interp;
+  a.src = grid;
+  a.dst = out;
+  a.wsort = wsort;
+  a.lbase = lbase;
+  a.perm = perm;
+  a.geoms = (const Geom*)geoms;
+  a.items = (const Item*)items;
+  a.nitems = nitems;
+  a.ccoils = ccoils;
+  a.nchunks = nchunks;
+  a.stream = (hipStream_t)stream;
+  return dispatch(ndim, P, a);
+  MRA_TRY_END
 }
 
-int mra_interp_grad(int ndim, int P, const void* grid, const void* a, float* grad_t, const float* tsort,
-                    const int* perm, const void* geoms, const void* items, int nitems, void* stream) {
-  LaunchArgs la{grid, grad_t, a, tsort, perm, (const Geom*)geoms, (const Item*)items, nitems,
-                (hipStream_t)stream};
-  return dispatch(ndim, P, Op::Grad, la);
+int mra_grid_partial(int ndim, int P, const void* ksp, const float* dcf, const float* wsort, const int* lbase,
+                     const int* perm, const void* geoms, const void* items, int nitems, int ccoils, int nchunks,
+                     void* scratch, void* stream) {
+  MRA_TRY_BEGIN
+  if (ccoils < 1 || ccoils > mra_max_coil_chunk(ndim, P)) MRA_FAIL("grid: coil chunk exceeds the LDS budget");
+  LaunchArgs a{};
+  a.op = Op::GridPartial;
+  a.src = ksp;
+  a.aux = dcf;
+  a.dst = scratch;
+  a.wsort = wsort;
+  a.lbase = lbase;
+  a.perm = perm;
+  a.geoms = (const Geom*)geoms;
+  a.items = (const Item*)items;
+  a.nitems = nitems;
+  a.ccoils = ccoils;
+  a.nchunks = nchunks;
+  a.stream = (hipStream_t)stream;
+  return dispatch(ndim, P, a);
+  MRA_TRY_END
+}
+
+int mra_grid_reduce(int ndim, int P, void* grid, const void* scratch, const void* geoms, int prob,
+                    const int* tile_items, int ntiles, int nbatch, int accumulate, void* stream) {
+  MRA_TRY_BEGIN
+  LaunchArgs a{};
+  a.op = Op::GridReduce;
+  a.src = scratch;
+  a.dst = grid;
+  a.geoms = (const Geom*)geoms;
+  a.prob = prob;
+  a.tile_items = tile_items;
+  a.ntiles = ntiles;
+  a.nbatch = nbatch;
+  a.accumulate = accumulate;
+  a.stream = (hipStream_t)stream;
+  return dispatch(ndim, P, a);
+  MRA_TRY_END
+}
+
+int mra_interp_grad(int ndim, int P, const void* grid, const void* a_, float* grad_t, const float* tsort,
+                    const int* perm, const void* geoms, const void* items, int nitems, int max_item_samples,
+                    void* stream) {
+  MRA_TRY_BEGIN
+  LaunchArgs a{};
+  a.op = Op::Grad;
+  a.src = grid;
+  a.aux = a_;
+  a.dst = grad_t;
+  a.tsort = tsort;
+  a.perm = perm;
+  a.geoms = (const Geom*)geoms;
+  a.items = (const Item*)items;
+  a.nitems = nitems;
+  a.nchunks = (max_item_samples + kThreads * kGradK - 1) / (kThreads * kGradK);
+  a.stream = (hipStream_t)stream;
+  return dispatch(ndim, P, a);
+  MRA_TRY_END
 }
 
 }  // extern "C"

@@ -72,14 +72,15 @@ def calc_toep_kernel_helper(nufft_adj_os: callable, trj: torch.Tensor, weights: 
     return fft(img, dim=tuple(range(-d, 0)))
 
 
-def _fftn_(x: torch.Tensor, d: int) -> torch.Tensor:
-    """Unnormalised forward FFT over the last d dims, in place."""
-    return fft_torch.fftn(x, dim=tuple(range(-d, 0)), out=x)
+def _fftn(x: torch.Tensor, d: int) -> torch.Tensor:
+    """Unnormalised forward FFT over the last d dims (rocFFT, out of place: torch's
+    ``out=`` form adds a full copy of the result)."""
+    return fft_torch.fftn(x, dim=tuple(range(-d, 0)))
 
 
-def _ifftn_(x: torch.Tensor, d: int) -> torch.Tensor:
-    """Unnormalised inverse FFT (no 1/n) over the last d dims, in place."""
-    return fft_torch.ifftn(x, dim=tuple(range(-d, 0)), norm="forward", out=x)
+def _ifftn(x: torch.Tensor, d: int) -> torch.Tensor:
+    """Unnormalised inverse FFT (no 1/n) over the last d dims, out of place."""
+    return fft_torch.ifftn(x, dim=tuple(range(-d, 0)), norm="forward")
 
 
 class NUFFT(nn.Module):
@@ -133,9 +134,9 @@ class NUFFT(nn.Module):
             kern = toeplitz_unshifted(toeplitz_kernels[n])
             x = img[n].reshape(B, *self.im_size).to(complex_dtype)
             g = ops.pad_expand(x, self.im_size, im_size_os, center=cen)
-            _fftn_(g, d)
+            g = _fftn(g, d)
             ops.cmul_bcast(g, kern)
-            _ifftn_(g, d)
+            g = _ifftn(g, d)
             ops.crop_reduce(g, self.im_size, im_size_os, B, center=cen, out=out[n])
         return out.reshape(N, *img_batch, *self.im_size)
 
@@ -261,7 +262,7 @@ class hip_nufft(NUFFT):
         lead = img.shape[:-d]
         x = img.reshape(-1, *im).to(complex_dtype)
         g = ops.pad_expand(x, im, G, apod=self.apod_table(img.device, im), scale=1.0 / math.sqrt(math.prod(im)))
-        _fftn_(g, d)
+        g = _fftn(g, d)
         g = fft_torch.fftshift(g, dim=tuple(range(-d, 0)))
         return g.reshape(*lead, *G)
 
@@ -292,7 +293,7 @@ class hip_nufft(NUFFT):
         lead = ksp.shape[1: ksp.ndim - len(trj_batch)]
         B = int(math.prod(lead))
         G = self.os_shape()
-        out = torch.zeros((N, B, *G), dtype=complex_dtype, device=ksp.device)
+        out = torch.empty((N, B, *G), dtype=complex_dtype, device=ksp.device)
         src = ksp.to(complex_dtype).reshape(N, B, M)
         for i in range(N):
             plan = self.plan(trj[i], grid_size=G, centred=True)
@@ -306,8 +307,7 @@ class hip_nufft(NUFFT):
         G = tuple(ksp_os.shape[-d:])
         lead = ksp_os.shape[:-d]
         g = fft_torch.ifftshift(ksp_os.to(complex_dtype), dim=tuple(range(-d, 0))).contiguous()
-        g = g.reshape(-1, *G)
-        _ifftn_(g, d)
+        g = _ifftn(g.reshape(-1, *G), d)
         img = ops.crop_reduce(g, self.im_size, G, g.shape[0], apod=self.apod_table(ksp_os.device),
                               scale=1.0 / math.sqrt(math.prod(self.im_size)))
         return img.reshape(*lead, *self.im_size)
@@ -330,7 +330,7 @@ class hip_nufft(NUFFT):
             plan = self.plan(trj[n])
             g = ops.pad_expand(x[n if x.shape[0] > 1 else 0], self.im_size, G, apod=apod,
                                scale=1.0 / math.sqrt(math.prod(self.im_size)))
-            _fftn_(g, d)
+            g = _fftn(g, d)
             ops.interp(g, plan, out_scale=1.0 / self.width ** d, out=out[n])
         return out.reshape(N, *img_batch, *trj_batch)
 
@@ -350,9 +350,9 @@ class hip_nufft(NUFFT):
         scale = 1.0 / (self.width ** d * math.sqrt(math.prod(self.im_size)))
         for n in range(N):
             plan = self.plan(trj[n])
-            g = torch.zeros((B, *G), dtype=complex_dtype, device=ksp.device)
+            g = torch.empty((B, *G), dtype=complex_dtype, device=ksp.device)
             ops.gridding(y[n if y.shape[0] > 1 else 0], plan, g)
-            _ifftn_(g, d)
+            g = _ifftn(g, d)
             ops.crop_reduce(g, self.im_size, G, B, apod=apod, scale=scale, out=out[n])
         return out.reshape(N, *ksp_batch, *self.im_size)
 

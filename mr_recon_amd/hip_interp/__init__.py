@@ -107,7 +107,7 @@ def grid(vals, locs, grid_size, width, kernel: str = "kaiser_bessel", norm: str 
     vflat = vc.reshape(nbatch, -1)
     if plan is None:
         plan = _plan(locs, grid_size, width, _beta(kernel_params))
-    out = torch.zeros((nbatch, *grid_size), dtype=torch.complex64, device=vals.device)
+    out = torch.empty((nbatch, *grid_size), dtype=torch.complex64, device=vals.device)
     ops.gridding(vflat, plan, out)
     out = out.reshape(*batch_shape, *grid_size)
     return out if was_complex else out.real.contiguous()
@@ -140,7 +140,7 @@ class InterpolateFn(Function):
         batch_shape = grad_out.shape[: grad_out.ndim - (len(ctx.locs_shape) - 1)]
         nb = int(math.prod(batch_shape))
         if ctx.needs_input_grad[0]:
-            g = torch.zeros((nb, *ctx.grid_size), dtype=torch.complex64, device=grad_out.device)
+            g = torch.empty((nb, *ctx.grid_size), dtype=torch.complex64, device=grad_out.device)
             ops.gridding(gc.reshape(nb, -1), plan, g)
             g = g.reshape(*batch_shape, *ctx.grid_size)
             g_vals = g if ctx.vals_complex else g.real

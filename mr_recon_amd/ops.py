@@ -38,17 +38,21 @@ def interp(grid: torch.Tensor, plan: SamplePlan, out_scale: float = 1.0, out: to
     elif out.numel() != B * plan.npts or not out.is_contiguous():
         raise ValueError("interp: out must be contiguous with B*M elements")
     g = geom if geom is not None else plan.geom(B, plan.npts, out_scale)
-    check(lib().mra_interp(D, plan.P, ptr(grid), ptr(out), ptr(plan.tsort), ptr(plan.perm), ptr(g),
-                           ptr(plan.items), plan.max_items, stream_ptr(grid.device)))
+    cc = plan.coil_chunk(B)
+    nch = -(-B // cc)
+    check(lib().mra_interp(D, plan.P, ptr(grid), ptr(out), ptr(plan.wsort), ptr(plan.lbase), ptr(plan.perm),
+                           ptr(g), ptr(plan.items), plan.nitems, cc, nch, stream_ptr(grid.device)))
     return out
 
 
 def gridding(ksp: torch.Tensor, plan: SamplePlan, out: torch.Tensor, dcf: torch.Tensor | None = None,
-             out_scale: float = 1.0) -> torch.Tensor:
-    """Scatter-accumulate: samples (B, M) complex64 -> ``out`` (B, *G), added in place.
+             out_scale: float = 1.0, accumulate: bool = False, scratch: torch.Tensor | None = None) -> torch.Tensor:
+    """Adjoint of ``interp``: samples (B, M) complex64 -> grid ``out`` (B, *G).
 
-    ``out`` must be zeroed by the caller when a fresh grid is wanted.
-    ``dcf`` (M,) float32 multiplies each sample first.
+    Every point of ``out`` is written (``accumulate=True`` adds to its contents
+    instead), so ``out`` needs no zeroing. ``dcf`` (M,) float32 multiplies each
+    sample first. ``scratch`` (optional) holds the per-item partial subgrids,
+    ``gridding_scratch_elems(plan, B)`` complex64 elements.
     """
     _lib.require_device(ksp, out, dcf)
     ksp = _c64(ksp, "ksp")
@@ -65,9 +69,24 @@ def gridding(ksp: torch.Tensor, plan: SamplePlan, out: torch.Tensor, dcf: torch.
             raise ValueError("dcf must be float32 with one entry per sample")
         dcf = dcf.contiguous()
     g = plan.geom(B, plan.npts, out_scale)
-    check(lib().mra_grid(D, plan.P, ptr(out), ptr(ksp), ptr(dcf), ptr(plan.tsort), ptr(plan.perm), ptr(g),
-                         ptr(plan.items), plan.max_items, stream_ptr(out.device)))
+    cc = plan.coil_chunk(B)
+    nch = -(-B // cc)
+    need = gridding_scratch_elems(plan, B)
+    if scratch is None:
+        scratch = torch.empty(need, dtype=torch.complex64, device=out.device)
+    elif scratch.numel() < need or scratch.dtype != torch.complex64 or not scratch.is_contiguous():
+        raise ValueError(f"gridding scratch must be contiguous complex64 with >= {need} elements")
+    st = stream_ptr(out.device)
+    check(lib().mra_grid_partial(D, plan.P, ptr(ksp), ptr(dcf), ptr(plan.wsort), ptr(plan.lbase), ptr(plan.perm),
+                                 ptr(g), ptr(plan.items), plan.nitems, cc, nch, ptr(scratch), st))
+    check(lib().mra_grid_reduce(D, plan.P, ptr(out), ptr(scratch), ptr(g), 0, ptr(plan.tile_items), plan.ntiles, B,
+                                int(accumulate), st))
     return out
+
+
+def gridding_scratch_elems(plan: SamplePlan, nbatch: int) -> int:
+    """complex64 elements of gridding scratch: one subgrid per (work item, batch entry)."""
+    return max(1, plan.nitems * nbatch * plan.subgrid)
 
 
 def interp_grad(grid: torch.Tensor, a: torch.Tensor, plan: SamplePlan, out_scale: float = 1.0) -> torch.Tensor:
@@ -85,7 +104,7 @@ def interp_grad(grid: torch.Tensor, a: torch.Tensor, plan: SamplePlan, out_scale
     gt = torch.empty((plan.npts, D), dtype=torch.float32, device=grid.device)
     g = plan.geom(B, plan.npts, out_scale)
     check(lib().mra_interp_grad(D, plan.P, ptr(grid), ptr(a), ptr(gt), ptr(plan.tsort), ptr(plan.perm), ptr(g),
-                                ptr(plan.items), plan.max_items, stream_ptr(grid.device)))
+                                ptr(plan.items), plan.nitems, plan.S, stream_ptr(grid.device)))
     return gt
 
 

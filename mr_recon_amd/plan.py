@@ -28,6 +28,13 @@ from ._lib import check, lib, ptr, stream_ptr
 CHUNK_LEN = 4096  # samples per binning chunk (one wave walks one chunk in order)
 
 
+def default_samples_per_item(npts: int) -> int:
+    """Work-item size: about a thousand items per trajectory, 256..8192 samples each."""
+    s = -(-npts // 1024)
+    s = -(-s // 256) * 256
+    return int(min(max(s, 256), 8192))
+
+
 def kernel_points(width: float) -> int:
     """Grid points per dimension a KB patch of width ``width`` can touch."""
     return int(math.floor(float(width))) + 1
@@ -56,7 +63,7 @@ class SamplePlan:
     wrap : per-dim integer added to a grid index before it wraps mod G. The
         NUFFT uses ``-(G//2)`` so kernels address the unshifted FFT output.
     tile : tile edge in grid points (default per dimensionality)
-    samples_per_item : max samples per work item (<= 512)
+    samples_per_item : max samples per work item (default: about 1k items)
     """
 
     def __init__(self, coords: torch.Tensor, grid_size, width: float, beta: float, *, scale=None, shift=None,
@@ -85,10 +92,9 @@ class SamplePlan:
         self.tile = int(tile or L.mra_default_tile(D))
         self.ntile = [(g + self.tile - 1) // self.tile for g in self.grid_size]
         self.ntiles = int(math.prod(self.ntile))
-        spi_max = L.mra_samples_per_item()
-        self.S = int(samples_per_item or spi_max)
-        if not 1 <= self.S <= spi_max:
-            raise ValueError(f"samples_per_item must be in 1..{spi_max}")
+        self.S = int(samples_per_item or default_samples_per_item(self.npts))
+        if self.S < 1:
+            raise ValueError("samples_per_item must be positive")
         self.wrap = tuple(int(w) for w in (wrap if wrap is not None else (0,) * D))
 
         flat = coords.reshape(-1, D).contiguous()
@@ -119,10 +125,26 @@ class SamplePlan:
         self.tile_start = torch.empty(nkeys + 1, dtype=torch.int32, device=dev)
         check(L.mra_bin_items(ptr(scanned), nchunks.value, self.ntiles, self.S, 0, ptr(nitems_ws), ptr(item_off),
                               ptr(scan_ws2), ptr(self.items), self.max_items, ptr(self.tile_start), st))
+        # item_off[t] is the first work item of tile t (t = ntiles: non-finite samples)
+        self.tile_items = item_off
+        # KB weights (D x P per sorted sample) and subgrid offsets, once per trajectory
+        self.wsort = torch.empty(self.npts * D * self.P, dtype=torch.float32, device=dev)
+        self.lbase = torch.empty(self.npts, dtype=torch.int32, device=dev)
+        check(L.mra_plan_weights(D, self.P, ptr(self.tsort), self.npts, _lib.ints(self.grid_size),
+                                 _lib.ints(self.wrap), self.width, self.beta, ptr(self.wsort), ptr(self.lbase), st))
+        # The only host synchronisation of a plan: the number of work items, which
+        # sizes the launches and the gridding scratch.
+        self.nitems = int(item_off[nkeys].item())
+        self.subgrid = int(L.mra_subgrid_points(D, self.P))
+        self.max_coil_chunk = int(L.mra_max_coil_chunk(D, self.P))
         # The workspaces go back to the caching allocator here; PyTorch only
         # hands them out again to work ordered after these kernels on this stream.
-        del hist, scanned, scan_ws, nitems_ws, item_off, scan_ws2, flat
+        del hist, scanned, scan_ws, nitems_ws, scan_ws2, flat
         self._geoms: dict[GeomKey, torch.Tensor] = {}
+
+    def coil_chunk(self, nbatch: int) -> int:
+        """Coils per block: as many subgrids as the per-block LDS budget holds."""
+        return max(1, min(nbatch, self.max_coil_chunk))
 
     def geom(self, nbatch: int, ksp_stride: int | None = None, out_scale: float = 1.0,
              grid_stride: int | None = None) -> torch.Tensor:
@@ -134,7 +156,8 @@ class SamplePlan:
         if g is None:
             g = make_geoms([dict(grid_off=0, grid_stride=gstride, ksp_off=0, ksp_stride=kstride, samp_off=0,
                                  nbatch=int(nbatch), npts=self.npts, grid_size=self.grid_size, tile=self.tile,
-                                 wrap=self.wrap, beta=self.beta, width=self.width, out_scale=out_scale)],
+                                 wrap=self.wrap, beta=self.beta, width=self.width, out_scale=out_scale,
+                                 item_base=0, scratch_off=0, tile_items_off=0)],
                            self.device)
             self._geoms[key] = g
         return g
@@ -149,6 +172,7 @@ def make_geoms(entries: list[dict], device) -> torch.Tensor:
         D = len(e["grid_size"])
         check(L.mra_geom_fill(C.addressof(buf) + i * gsz, e["grid_off"], e["grid_stride"], e["ksp_off"],
                               e["ksp_stride"], e["samp_off"], e["nbatch"], e["npts"], D, _lib.ints(e["grid_size"]),
-                              e["tile"], _lib.ints(e["wrap"]), e["beta"], e["width"], e["out_scale"]))
+                              e["tile"], _lib.ints(e["wrap"]), e["beta"], e["width"], e["out_scale"],
+                              e.get("item_base", 0), e.get("scratch_off", 0), e.get("tile_items_off", 0)))
     host = torch.frombuffer(bytearray(bytes(buf)), dtype=torch.uint8)
     return host.to(device)

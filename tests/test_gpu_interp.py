@@ -95,6 +95,58 @@ def test_grid_matches_oracle(dev, D, G, width, B):
     assert rel_l2(out, ref) < TOL
 
 
+@pytest.mark.parametrize("D,G,width", [(1, 3, 4.0), (2, 5, 4.0), (3, 5, 4.0), (3, 11, 6.0), (2, 7, 8.0)])
+def test_grid_tiny_grids(dev, D, G, width):
+    """Grids smaller than one subgrid: a partial subgrid wraps onto the same grid point
+    several times and every wrap must be summed."""
+    from mr_recon_amd.hip_interp import grid
+
+    g = torch.Generator().manual_seed(G)
+    k = (torch.rand((300, D), generator=g) - 0.3) * 2 * G
+    vals = torch.randn((2, 300), dtype=torch.complex64, generator=g)
+    out = grid(vals.to(dev), k.to(dev), (G,) * D, width, kernel_params={"beta": 5.0})
+    assert rel_l2(out, interp_ref.grid(vals, k, (G,) * D, width, beta=5.0)) < TOL
+
+
+@pytest.mark.parametrize("D,G", [(3, 50), (3, 37), (2, 100), (2, 70), (1, 700)])
+def test_grid_partial_last_tile(dev, D, G):
+    """G not a multiple of the tile edge: the last tile's subgrid wraps into tiles 0 and 1."""
+    from mr_recon_amd.hip_interp import grid, ungrid
+
+    g = torch.Generator().manual_seed(G)
+    k = torch.rand((6000, D), generator=g) * G
+    k[:200] = G - torch.rand((200, D), generator=g) * 3  # crowd the wrap seam
+    vals = torch.randn((3, 6000), dtype=torch.complex64, generator=g)
+    out = grid(vals.to(dev), k.to(dev), (G,) * D, 4.0, kernel_params={"beta": 7.0})
+    assert rel_l2(out, interp_ref.grid(vals, k, (G,) * D, 4.0, beta=7.0)) < TOL
+    x = torch.randn((3, *(G,) * D), dtype=torch.complex64, generator=g)
+    y = ungrid(x.to(dev), k.to(dev), 4.0, kernel_params={"beta": 7.0})
+    assert rel_l2(y, interp_ref.ungrid(x, k, 4.0, beta=7.0)) < TOL
+
+
+def test_plan_items_cover_every_sample(dev):
+    """Work items: per tile contiguous, each at most S samples, together exactly the sorted range."""
+    from mr_recon_amd.plan import SamplePlan
+
+    k = torch.from_numpy(edge_coords((60, 60), 20000, seed=3))
+    k[7, 0] = float("nan")
+    plan = SamplePlan(k.to(dev), (60, 60), 4.0, 6.0, samples_per_item=300)
+    items = plan.items[: plan.nitems].cpu().numpy()
+    ti = plan.tile_items.cpu().numpy()
+    start = plan.tile_start.cpu().numpy()
+    assert ti[-1] == plan.nitems
+    cover = np.zeros(plan.npts, dtype=np.int64)
+    for t in range(plan.ntiles + 1):
+        its = items[ti[t]:ti[t + 1]]
+        assert np.all(its[:, 1] == (t if t < plan.ntiles else -1))
+        assert np.all((its[:, 3] > 0) & (its[:, 3] <= 300))
+        if len(its):
+            assert its[0, 2] == start[t] and its[-1, 2] + its[-1, 3] == start[t + 1]
+        for it in its:
+            cover[it[2]:it[2] + it[3]] += 1
+    assert np.all(cover == 1)
+
+
 def test_ungrid_bruteforce_tiny(dev):
     """Tiny grids (G smaller than the patch) against the pure-Python definition."""
     from mr_recon_amd.hip_interp import ungrid

@@ -1,0 +1,107 @@
+"""GPU parity: fused SENSE linop (forward / adjoint / Toeplitz normal), CG and CG-SENSE vs the oracle."""
+
+import pytest
+import torch
+
+from helpers import cdot, rel_l2
+from mr_recon_amd import synth
+from oracle.nufft_ref import TritonNufftRef
+from oracle.sense_ref import SenseRef, cg_sense_recon, conjugate_gradient
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+CASES = [
+    ((48, 48), 8, 1.25, 4, lambda: synth.spiral_2d(8, 500, 48)),
+    ((40, 40), 4, 2.0, 6, lambda: synth.radial_2d(30, 80, 40)),
+    ((16, 16, 16), 6, 1.25, 4, lambda: synth.kooshball_3d(80, 32, 16)),
+]
+
+
+def _ops(case, dev, toeplitz=False, dcf=True):
+    from mr_recon_amd.fourier import hip_nufft
+    from mr_recon_amd.linops import sense_linop
+
+    im, C, os_, w, mk = CASES[case]
+    trj = mk()
+    mps = synth.coil_maps(C, im, seed=case)
+    d = torch.rand(trj.shape[:-1], generator=torch.Generator().manual_seed(case)) + 0.5 if dcf else None
+    A = sense_linop(trj.to(dev), mps.to(dev), dcf=None if d is None else d.to(dev),
+                    nufft=hip_nufft(im, oversamp=os_, width=w), use_toeplitz=toeplitz)
+    R = SenseRef(trj, mps, dcf=d, nufft=TritonNufftRef(im, oversamp=os_, width=w), use_toeplitz=toeplitz)
+    return A, R, im
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_sense_forward_adjoint(dev, case):
+    A, R, im = _ops(case, dev)
+    x = synth.random_image(im, seed=case)
+    y = A.forward(x.to(dev))
+    y_ref = R.forward(x)
+    assert y.shape == y_ref.shape
+    assert rel_l2(y, y_ref) < TOL
+    k = synth.random_image(y_ref.shape, seed=case + 9)
+    z = A.adjoint(k.to(dev))
+    assert z.shape == im
+    assert rel_l2(z, R.adjoint(k)) < TOL
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_sense_toeplitz_normal(dev, case):
+    A, R, im = _ops(case, dev, toeplitz=True)
+    assert rel_l2(A.toep_kerns, R.toep_kerns) < TOL
+    x = synth.random_image(im, seed=case + 3)
+    assert rel_l2(A.normal(x.to(dev)), R.normal(x)) < TOL
+
+
+def test_sense_normal_is_hermitian(dev):
+    A, _, im = _ops(0, dev, toeplitz=False)
+    x = synth.random_image(im, seed=1).to(dev)
+    v = synth.random_image(im, seed=2).to(dev)
+    a, b = cdot(v, A.normal(x)), cdot(A.normal(v), x)
+    assert abs(a - b) / abs(a) < 1e-5
+
+
+@pytest.mark.parametrize("toeplitz", [False, True])
+def test_cg_sense_vs_oracle(dev, toeplitz):
+    """CG-SENSE (10 iterations, lambda 1e-3, fixed max_eigen) against the FP64 oracle, with the
+    fp32 oracle's own deviation from FP64 as the scale of acceptable rounding."""
+    from mr_recon_amd.fourier import hip_nufft
+    from mr_recon_amd.linops import sense_linop
+    from mr_recon_amd.recons import CG_SENSE_recon
+
+    im = (48, 48)
+    trj = synth.spiral_2d(12, 400, 48)
+    mps = synth.coil_maps(6, im, seed=3)
+    x = synth.phantom_2d(48)
+    R64 = SenseRef(trj, mps.to(torch.complex128), nufft=TritonNufftRef(im, 1.25, 4, dtype=torch.float64),
+                   use_toeplitz=toeplitz)
+    R32 = SenseRef(trj, mps, nufft=TritonNufftRef(im, 1.25, 4), use_toeplitz=toeplitz)
+    y = R64.forward(x.to(torch.complex128))
+    A = sense_linop(trj.to(dev), mps.to(dev), nufft=hip_nufft(im, 1.25, 4), use_toeplitz=toeplitz)
+    kw = dict(max_iter=10, lamda_l2=1e-3, max_eigen=2.0, tolerance=0.0)
+    got = CG_SENSE_recon(A, y.to(torch.complex64).to(dev), verbose=False, **kw)
+    ref64 = cg_sense_recon(R64, y, **kw)
+    ref32 = cg_sense_recon(R32, y.to(torch.complex64), **kw)
+    e32 = rel_l2(ref32, ref64)
+    assert rel_l2(got, ref64) < max(1e-5, 5 * e32)
+
+
+def test_cg_small_spd(dev):
+    """Fused CG kernels on a dense SPD operator vs torch.linalg.solve and the oracle CG."""
+    from mr_recon_amd.algs import conjugate_gradient as cg
+
+    g = torch.Generator().manual_seed(0)
+    M = torch.randn(64, 64, dtype=torch.complex128, generator=g)
+    H = M.conj().T @ M / 256 + torch.eye(64, dtype=torch.complex128)  # condition number ~ 5
+    b = torch.randn(64, dtype=torch.complex128, generator=g)
+    Hd = H.to(torch.complex64).to(dev)
+    x = cg(lambda v: Hd @ v, b.to(torch.complex64).to(dev), num_iters=60, lamda_l2=0.1, tolerance=0.0)
+    ex = torch.linalg.solve(H + 0.1 * torch.eye(64, dtype=torch.complex128), b)
+    assert rel_l2(x, ex) < 1e-4
+    ref = conjugate_gradient(lambda v: H @ v, b, num_iters=5, lamda_l2=0.1, tolerance=0.0)
+    x5 = cg(lambda v: Hd @ v, b.to(torch.complex64).to(dev), num_iters=5, lamda_l2=0.1, tolerance=0.0)
+    assert rel_l2(x5, ref) < 1e-5
+    resids, xs = cg(lambda v: Hd @ v, b.to(torch.complex64).to(dev), num_iters=5, return_resids=True)
+    assert len(resids) == 5 and len(xs) == 6
