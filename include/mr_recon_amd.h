@@ -67,7 +67,7 @@ int mra_interp(int ndim, int P, const void* grid, void* out, const float* wsort,
 int mra_grid_partial(int ndim, int P, const void* ksp, const float* dcf, const float* wsort, const int* lbase,
                      const int* perm, const void* geoms, const void* items, int nitems, int ccoils, int nchunks,
                      void* scratch, void* stream);
-int mra_grid_reduce(int ndim, int P, void* grid, const void* scratch, const void* geoms, int prob,
+int mra_grid_reduce(int ndim, int P, void* grid, const void* scratch, const void* geoms, int prob0, int nprob,
                     const int* tile_items, int ntiles, int nbatch, int accumulate, void* stream);
 /* Gradient of Re<a, interp(grid, t)> with respect to the sample coordinates t.
  * No reference counterpart: InterpolateFn.backward returns None for locs
@@ -90,6 +90,15 @@ int mra_pad_expand(int ndim, const int* N, const int* G, const int* center, cons
 int mra_crop_reduce(int ndim, const int* N, const int* G, const int* center, const void* grid, long long nx,
                     const void* smaps, long long ns, long long s_stride, int conj_maps, const float* apod,
                     float scale, int accumulate, int roll_out, void* out, long long x_stride, void* stream);
+/* Batched problems of one image size (no reference counterpart: the reference loops
+ * over problems in Python). Grid q pads image img_index[q] times map q; image i reduces
+ * grids/maps coil_off[i] .. coil_off[i+1]-1. */
+int mra_pad_expand_batched(int ndim, const int* N, const int* G, const int* center, const void* x, long long nimg,
+                           const void* smaps, long long nq, const int* img_index, const float* apod, float scale,
+                           void* out, void* stream);
+int mra_crop_reduce_batched(int ndim, const int* N, const int* G, const int* center, const void* grid,
+                            long long nimg, const void* smaps, const int* coil_off, const float* apod, float scale,
+                            void* out, void* stream);
 /* Toeplitz PSF multiply (fourier.py:344, linops.py:1285-1287). */
 int mra_cmul_bcast(void* data, const void* k, long long nbatch, long long n, void* stream);
 
@@ -103,6 +112,13 @@ int mra_cg_init(const void* b, const void* ax, const void* x, void* r, void* p, 
                 double* part, double* sc, void* stream);
 int mra_cg_step(void* x, void* r, void* p, const void* ap, long long n, float lam, double* part, double* sc,
                 void* stream);
+/* nseg independent systems of seg_len elements (batched problems; the reference runs
+ * one conjugate_gradient per problem). */
+int mra_cg_seg_blocks(void);
+int mra_cg_init_seg(const void* b, const void* ax, const void* x, void* r, void* p, long long seg_len, int nseg,
+                    float lam, double* part, double* sc, void* stream);
+int mra_cg_step_seg(void* x, void* r, void* p, const void* ap, long long seg_len, int nseg, float lam,
+                    double* part, double* sc, void* stream);
 
 #ifdef __cplusplus
 }

@@ -46,15 +46,20 @@ SIGNATURES = {
     "mra_bin_items": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp]),
     "mra_interp": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "mra_grid_partial": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp]),
-    "mra_grid_reduce": (_i, [_i, _i, _vp, _vp, _vp, _i, _vp, _i, _i, _i, _vp]),
+    "mra_grid_reduce": (_i, [_i, _i, _vp, _vp, _vp, _i, _i, _vp, _i, _i, _i, _vp]),
     "mra_interp_grad": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
     "mra_pad_expand": (_i, [_i, _ip, _ip, _ip, _vp, _ll, _ll, _vp, _ll, _ll, _vp, _f, _vp, _vp]),
     "mra_crop_reduce": (_i, [_i, _ip, _ip, _ip, _vp, _ll, _vp, _ll, _ll, _i, _vp, _f, _i, _i, _vp, _ll, _vp]),
+    "mra_pad_expand_batched": (_i, [_i, _ip, _ip, _ip, _vp, _ll, _vp, _ll, _vp, _vp, _f, _vp, _vp]),
+    "mra_crop_reduce_batched": (_i, [_i, _ip, _ip, _ip, _vp, _ll, _vp, _vp, _vp, _f, _vp, _vp]),
     "mra_cmul_bcast": (_i, [_vp, _vp, _ll, _ll, _vp]),
     "mra_cg_partials": (_i, []),
+    "mra_cg_seg_blocks": (_i, []),
     "mra_cdot_re": (_i, [_vp, _vp, _ll, _f, _vp, _vp, _i, _vp]),
     "mra_cg_init": (_i, [_vp, _vp, _vp, _vp, _vp, _ll, _f, _vp, _vp, _vp]),
     "mra_cg_step": (_i, [_vp, _vp, _vp, _vp, _ll, _f, _vp, _vp, _vp]),
+    "mra_cg_init_seg": (_i, [_vp, _vp, _vp, _vp, _vp, _ll, _i, _f, _vp, _vp, _vp]),
+    "mra_cg_step_seg": (_i, [_vp, _vp, _vp, _vp, _ll, _i, _f, _vp, _vp, _vp]),
 }
 
 

@@ -117,6 +117,110 @@ __global__ __launch_bounds__(kRedThreads) void cg_init_kernel(const float2* __re
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// ---- segmented variants: nseg independent systems of seg_len elements each (batched
+// problems of one image size). blockIdx.y = segment; partials part[seg][kSegBlocks];
+// scalars sc[seg][8] with the same slots as above.
+constexpr int kSegBlocks = 32;
+
+__global__ __launch_bounds__(kRedThreads) void seg_cdot_partial_kernel(const float2* __restrict__ a,
+                                                                       const float2* __restrict__ b,
+                                                                       long long seg_len, float lam,
+                                                                       double* __restrict__ part) {
+  const long long base = (long long)blockIdx.y * seg_len;
+  double s = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < seg_len;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float2 av = a[base + i], bv = b[base + i];
+    const double bx = (double)bv.x + (double)lam * av.x, by = (double)bv.y + (double)lam * av.y;
+    s += (double)av.x * bx + (double)av.y * by;
+  }
+  const double t = block_sum(s);
+  if (threadIdx.x == 0) part[blockIdx.y * kSegBlocks + blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kRedThreads) void seg_reduce_final_kernel(const double* __restrict__ part, int nparts,
+                                                                       double* __restrict__ sc, int slot, int mode) {
+  const int seg = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[seg * kSegBlocks + i];
+  const double t = block_sum(s);
+  if (threadIdx.x == 0) {
+    double* c = sc + seg * 8;
+    if (mode == 0) {
+      c[slot] = t;
+    } else {
+      c[3] = c[0] > 0.0 ? t / c[0] : 0.0;
+      c[2] = c[0];
+      c[0] = t;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kRedThreads) void seg_cg_xr_kernel(float2* __restrict__ x, float2* __restrict__ r,
+                                                                const float2* __restrict__ p,
+                                                                const float2* __restrict__ ap, long long seg_len,
+                                                                float lam, const double* __restrict__ sc,
+                                                                double* __restrict__ part) {
+  const double* c = sc + blockIdx.y * 8;
+  const float alpha = c[1] > 0.0 ? (float)(c[0] / c[1]) : 0.0f;
+  const long long base = (long long)blockIdx.y * seg_len;
+  double s = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < seg_len;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long j = base + i;
+    const float2 pv = p[j], av = ap[j];
+    float2 xv = x[j], rv = r[j];
+    xv.x += alpha * pv.x;
+    xv.y += alpha * pv.y;
+    rv.x -= alpha * (av.x + lam * pv.x);
+    rv.y -= alpha * (av.y + lam * pv.y);
+    x[j] = xv;
+    r[j] = rv;
+    s += (double)rv.x * rv.x + (double)rv.y * rv.y;
+  }
+  const double t = block_sum(s);
+  if (threadIdx.x == 0) part[blockIdx.y * kSegBlocks + blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kRedThreads) void seg_cg_p_kernel(float2* __restrict__ p, const float2* __restrict__ r,
+                                                               long long seg_len, const double* __restrict__ sc) {
+  const float beta = (float)sc[blockIdx.y * 8 + 3];
+  const long long base = (long long)blockIdx.y * seg_len;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < seg_len;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float2 rv = r[base + i], pv = p[base + i];
+    p[base + i] = make_float2(rv.x + beta * pv.x, rv.y + beta * pv.y);
+  }
+}
+
+__global__ __launch_bounds__(kRedThreads) void seg_cg_init_kernel(const float2* __restrict__ b,
+                                                                  const float2* __restrict__ ax,
+                                                                  const float2* __restrict__ x,
+                                                                  float2* __restrict__ r, float2* __restrict__ p,
+                                                                  long long seg_len, float lam,
+                                                                  double* __restrict__ part) {
+  const long long base = (long long)blockIdx.y * seg_len;
+  double s = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < seg_len;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long j = base + i;
+    const float2 bv = b[j], av = ax[j], xv = x[j];
+    const float2 rv = make_float2(bv.x - (av.x + lam * xv.x), bv.y - (av.y + lam * xv.y));
+    r[j] = rv;
+    p[j] = rv;
+    s += (double)rv.x * rv.x + (double)rv.y * rv.y;
+  }
+  const double t = block_sum(s);
+  if (threadIdx.x == 0) part[blockIdx.y * kSegBlocks + blockIdx.x] = t;
+}
+
+static unsigned seg_blocks(long long seg_len) {
+  long long b = (seg_len + kRedThreads - 1) / kRedThreads;
+  if (b > kSegBlocks) b = kSegBlocks;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
 static unsigned red_blocks(long long n) {
   long long b = (n + kRedThreads - 1) / kRedThreads;
   if (b > kRedBlocks) b = kRedBlocks;
@@ -131,6 +235,39 @@ using namespace mra;
 extern "C" {
 
 int mra_cg_partials() { return kRedBlocks; }
+int mra_cg_seg_blocks() { return kSegBlocks; }
+
+// Segmented: part holds nseg * mra_cg_seg_blocks() doubles, sc nseg * 8.
+int mra_cg_init_seg(const void* b, const void* ax, const void* x, void* r, void* p, long long seg_len, int nseg,
+                    float lam, double* part, double* sc, void* stream) {
+  MRA_TRY_BEGIN
+  if (seg_len <= 0 || nseg <= 0 || nseg > 65535) MRA_FAIL("cg_init_seg: bad segment sizes");
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned nb = seg_blocks(seg_len);
+  seg_cg_init_kernel<<<dim3(nb, nseg), kRedThreads, 0, st>>>((const float2*)b, (const float2*)ax,
+                                                               (const float2*)x, (float2*)r, (float2*)p, seg_len,
+                                                               lam, part);
+  seg_reduce_final_kernel<<<nseg, kRedThreads, 0, st>>>(part, (int)nb, sc, 0, 0);
+  MRA_CHECK_LAUNCH("cg_init_seg");
+  MRA_TRY_END
+}
+
+int mra_cg_step_seg(void* x, void* r, void* p, const void* ap, long long seg_len, int nseg, float lam,
+                    double* part, double* sc, void* stream) {
+  MRA_TRY_BEGIN
+  if (seg_len <= 0 || nseg <= 0 || nseg > 65535) MRA_FAIL("cg_step_seg: bad segment sizes");
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned nb = seg_blocks(seg_len);
+  const dim3 grd(nb, nseg);
+  seg_cdot_partial_kernel<<<grd, kRedThreads, 0, st>>>((const float2*)p, (const float2*)ap, seg_len, lam, part);
+  seg_reduce_final_kernel<<<nseg, kRedThreads, 0, st>>>(part, (int)nb, sc, 1, 0);
+  seg_cg_xr_kernel<<<grd, kRedThreads, 0, st>>>((float2*)x, (float2*)r, (const float2*)p, (const float2*)ap,
+                                                seg_len, lam, sc, part);
+  seg_reduce_final_kernel<<<nseg, kRedThreads, 0, st>>>(part, (int)nb, sc, 0, 1);
+  seg_cg_p_kernel<<<grd, kRedThreads, 0, st>>>((float2*)p, (const float2*)r, seg_len, sc);
+  MRA_CHECK_LAUNCH("cg_step_seg");
+  MRA_TRY_END
+}
 
 // sc[slot] = Re <a, b + lam a>  (FP64)
 int mra_cdot_re(const void* a, const void* b, long long n, float lam, double* part, double* sc, int slot,

@@ -271,16 +271,21 @@ __global__ __launch_bounds__(kThreads) void grid_partial_kernel(const float2* __
   for (int e = tid; e < nc * TS::SUBN; e += kThreads) dst[e] = src[e];
 }
 
-// One block per (output tile, batch entry): sums the covering subgrids in a fixed order.
+// One block per (problem, output tile, batch entry): sums the covering subgrids in a
+// fixed order. blockIdx.x = (prob - prob0) * ntiles + tile; problems of one launch
+// share the tile count.
 template <int D, int P, int T>
 __global__ __launch_bounds__(kThreads) void grid_reduce_kernel(float2* __restrict__ grid,
                                                                const float2* __restrict__ scratch,
-                                                               const Geom* __restrict__ geoms, int prob,
-                                                               const int* __restrict__ tile_items, int accumulate) {
+                                                               const Geom* __restrict__ geoms, int prob0,
+                                                               int ntiles, const int* __restrict__ tile_items,
+                                                               int accumulate) {
   using TS = TileShape<D, P, T>;
+  const int prob = prob0 + blockIdx.x / ntiles;
+  const int tile = blockIdx.x - (prob - prob0) * ntiles;
   const Geom g = geoms[prob];
-  const int tile = blockIdx.x;
   const int b = blockIdx.y;
+  if (b >= g.nbatch) return;
   int tc[D];
   tile_coords<D, T>(tile, g, tc);
   // Candidate source tiles per dim: t, t-1, t-2 (mod ntile), those whose subgrid
@@ -560,7 +565,7 @@ struct LaunchArgs {
   const Item* items;
   int nitems;
   int ccoils, nchunks;  // coils per block, coil chunks
-  int prob, ntiles, nbatch, accumulate;
+  int prob, nprob, ntiles, nbatch, accumulate;
   const int* tile_items;
   int npts;
   hipStream_t stream;
@@ -595,14 +600,14 @@ static int launch_op(const LaunchArgs& a) {
       break;
     }
     case Op::GridReduce: {
-      if (a.ntiles <= 0 || a.nbatch <= 0) break;
+      if (a.ntiles <= 0 || a.nbatch <= 0 || a.nprob <= 0) break;
       if (a.nbatch > 65535) {
         set_last_error("grid_reduce: batch larger than 65535");
         return 1;
       }
-      dim3 grd((unsigned)a.ntiles, (unsigned)a.nbatch);
+      dim3 grd((unsigned)((long long)a.ntiles * a.nprob), (unsigned)a.nbatch);
       grid_reduce_kernel<D, P, T><<<grd, kThreads, 0, a.stream>>>((float2*)a.dst, (const float2*)a.src, a.geoms,
-                                                                  a.prob, a.tile_items, a.accumulate);
+                                                                  a.prob, a.ntiles, a.tile_items, a.accumulate);
       break;
     }
     case Op::Grad: {
@@ -781,7 +786,8 @@ int mra_grid_partial(int ndim, int P, const void* ksp, const float* dcf, const f
   MRA_TRY_END
 }
 
-int mra_grid_reduce(int ndim, int P, void* grid, const void* scratch, const void* geoms, int prob,
+// Problems prob0 .. prob0+nprob-1 (all with ntiles tiles); nbatch = largest batch among them.
+int mra_grid_reduce(int ndim, int P, void* grid, const void* scratch, const void* geoms, int prob0, int nprob,
                     const int* tile_items, int ntiles, int nbatch, int accumulate, void* stream) {
   MRA_TRY_BEGIN
   LaunchArgs a{};
@@ -789,7 +795,8 @@ int mra_grid_reduce(int ndim, int P, void* grid, const void* scratch, const void
   a.src = scratch;
   a.dst = grid;
   a.geoms = (const Geom*)geoms;
-  a.prob = prob;
+  a.prob = prob0;
+  a.nprob = nprob;
   a.tile_items = tile_items;
   a.ntiles = ntiles;
   a.nbatch = nbatch;

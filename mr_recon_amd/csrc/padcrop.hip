@@ -48,7 +48,7 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
 __global__ __launch_bounds__(256) void pad_expand_kernel(const float2* __restrict__ x,
                                                          const float2* __restrict__ smaps,
                                                          float2* __restrict__ out, PadCropParams p,
-                                                         long long nrows) {
+                                                         long long nrows, const int* __restrict__ img_index) {
   const int lane = threadIdx.x & 63;
   const long long row = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= nrows) return;
@@ -57,9 +57,10 @@ __global__ __launch_bounds__(256) void pad_expand_kernel(const float2* __restric
   const int g1 = (int)(row % G1);
   long long rr = row / G1;
   const int g0 = (int)(rr % G0);
-  rr /= G0;
-  const long long bs = rr % p.ns;
-  const long long bx = rr / p.ns;
+  rr /= G0;  // grid batch index q = bx * ns + bs
+  // batched problems: grid q belongs to image img_index[q] and uses map q
+  const long long bs = img_index ? rr : rr % p.ns;
+  const long long bx = img_index ? img_index[rr] : rr / p.ns;
   float2* orow = out + row * G2;
   const int n0 = grid_to_pixel(g0, N0, G0, p.c[0]), n1 = grid_to_pixel(g1, N1, G1, p.c[1]);
   if (n0 < 0 || n1 < 0) {
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(256) void pad_expand_kernel(const float2* __restric
 __global__ __launch_bounds__(256) void crop_reduce_kernel(const float2* __restrict__ grid,
                                                           const float2* __restrict__ smaps,
                                                           float2* __restrict__ out, PadCropParams p,
-                                                          long long nrows) {
+                                                          long long nrows, const int* __restrict__ coil_off) {
   const int lane = threadIdx.x & 63;
   const long long row = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (row >= nrows) return;
@@ -101,7 +102,10 @@ __global__ __launch_bounds__(256) void crop_reduce_kernel(const float2* __restri
   const long long bx = rr / N0;
   const int g0 = pmod(n0 - p.c[0], G0), g1 = pmod(n1 - p.c[1], G1);
   const long long gtot = (long long)G0 * G1 * G2;
-  const float2* grow = grid + (bx * p.ns) * gtot + ((long long)g0 * G1 + g1) * G2;
+  // grids (and maps) [q0, q1) reduce into image bx; batched problems give the range per image
+  const long long q0 = coil_off ? coil_off[bx] : bx * p.ns;
+  const long long q1 = coil_off ? coil_off[bx + 1] : q0 + p.ns;
+  const float2* grow = grid + q0 * gtot + ((long long)g0 * G1 + g1) * G2;
   const long long pix = ((long long)n0 * N1 + n1) * N2;
   const float a01 = p.scale * (p.apod[0] ? p.apod[0][n0] : 1.0f) * (p.apod[1] ? p.apod[1][n1] : 1.0f);
   long long orow;
@@ -114,10 +118,10 @@ __global__ __launch_bounds__(256) void crop_reduce_kernel(const float2* __restri
   for (int n2 = lane; n2 < N2; n2 += 64) {
     const int g2 = pmod(n2 - p.c[2], G2);
     float2 acc = make_float2(0.f, 0.f);
-    for (long long bs = 0; bs < p.ns; ++bs) {
+    for (long long bs = 0; bs < q1 - q0; ++bs) {
       float2 v = grow[bs * gtot + g2];
       if (smaps) {
-        const float2 s = smaps[bs * p.s_stride + pix + n2];
+        const float2 s = smaps[((coil_off ? q0 : 0) + bs) * p.s_stride + pix + n2];
         v = p.conj_maps ? cmulc(v, s) : cmul(v, s);
       }
       acc.x += v.x;
@@ -199,8 +203,26 @@ int mra_pad_expand(int ndim, const int* N, const int* G, const int* center, cons
   const long long nrows = p.nx * p.ns * p.G[0] * p.G[1];
   const long long nblk = (nrows + 3) / 4;
   pad_expand_kernel<<<(unsigned)nblk, 256, 0, (hipStream_t)stream>>>((const float2*)x, (const float2*)smaps,
-                                                                   (float2*)out, p, nrows);
+                                                                   (float2*)out, p, nrows, nullptr);
   MRA_CHECK_LAUNCH("pad_expand");
+  MRA_TRY_END
+}
+
+// Batched problems of one image size: grid q (0 <= q < nq) = pad(apod * x[img_index[q]] * S[q]).
+int mra_pad_expand_batched(int ndim, const int* N, const int* G, const int* center, const void* x, long long nimg,
+                           const void* smaps, long long nq, const int* img_index, const float* apod, float scale,
+                           void* out, void* stream) {
+  MRA_TRY_BEGIN
+  PadCropParams p{};
+  if (fill_params(p, ndim, N, G, center, nimg, 1, 1, 1, apod, scale)) return 1;
+  const long long npix = (long long)p.N[0] * p.N[1] * p.N[2];
+  p.x_stride = npix;
+  p.s_stride = npix;
+  const long long nrows = nq * p.G[0] * p.G[1];
+  const long long nblk = (nrows + 3) / 4;
+  pad_expand_kernel<<<(unsigned)nblk, 256, 0, (hipStream_t)stream>>>((const float2*)x, (const float2*)smaps,
+                                                                   (float2*)out, p, nrows, img_index);
+  MRA_CHECK_LAUNCH("pad_expand_batched");
   MRA_TRY_END
 }
 
@@ -217,8 +239,27 @@ int mra_crop_reduce(int ndim, const int* N, const int* G, const int* center, con
   const long long nrows = p.nx * p.N[0] * p.N[1];
   const long long nblk = (nrows + 3) / 4;
   crop_reduce_kernel<<<(unsigned)nblk, 256, 0, (hipStream_t)stream>>>((const float2*)grid, (const float2*)smaps,
-                                                                    (float2*)out, p, nrows);
+                                                                    (float2*)out, p, nrows, nullptr);
   MRA_CHECK_LAUNCH("crop_reduce");
+  MRA_TRY_END
+}
+
+// Batched problems of one image size: image i = scale * apod * sum_{q in [coil_off[i], coil_off[i+1])}
+// conj(S[q]) * crop(grid[q]).
+int mra_crop_reduce_batched(int ndim, const int* N, const int* G, const int* center, const void* grid,
+                            long long nimg, const void* smaps, const int* coil_off, const float* apod, float scale,
+                            void* out, void* stream) {
+  MRA_TRY_BEGIN
+  PadCropParams p{};
+  if (fill_params(p, ndim, N, G, center, nimg, 1, 1, 1, apod, scale)) return 1;
+  const long long npix = (long long)p.N[0] * p.N[1] * p.N[2];
+  p.x_stride = npix;
+  p.s_stride = npix;
+  const long long nrows = p.nx * p.N[0] * p.N[1];
+  const long long nblk = (nrows + 3) / 4;
+  crop_reduce_kernel<<<(unsigned)nblk, 256, 0, (hipStream_t)stream>>>((const float2*)grid, (const float2*)smaps,
+                                                                    (float2*)out, p, nrows, coil_off);
+  MRA_CHECK_LAUNCH("crop_reduce_batched");
   MRA_TRY_END
 }
 

@@ -1,0 +1,199 @@
+"""Differentiable NUFFT / SENSE with a hand-written trajectory gradient.
+
+The reference differentiates only with respect to values: ``InterpolateFn`` and
+``InterpolateAdjointFn`` pair ungrid with grid and return ``None`` for the
+coordinates (triton_interp/interp.py:49-75, 130-155). Here each operator is a
+``torch.autograd.Function`` whose backward is the exact adjoint for the values
+and, for the trajectory, the derivative of the Kaiser-Bessel weights in the
+gather (``mra_interp_grad``):
+
+  forward  y_c = A(S_c x),   A = interp_t . FFT . pad . apod / sqrt(prod N),  interp / w^d
+     dL/dt_m = sum_c Re( conj(g_cm) * sum_p dW_p/dt * F_c[p] ) / w^d,   F_c = FFT(pad(apod S_c x))/sqrt(prod N)
+  adjoint  x = sum_c conj(S_c) A^H(dcf * y_c)
+     dL/dt_m = sum_c Re( conj(dcf_m y_cm) * sum_p dW_p/dt * H_c[p] ) / w^d,  H_c = FFT(pad(apod S_c gx))/sqrt(prod N)
+
+with t = k * (ceil(s N)/N) + ceil(s N)//2, so dL/dk = dL/dt * ceil(s N)/N. The
+oversampled grids F_c / H_c are recomputed in backward (one pad + FFT), not
+saved: a checkpointed recompute that keeps the forward's memory at the size of
+its output. ``unrolled_cg_sense`` runs CG-SENSE iterations as differentiable
+torch ops around these operators, each iteration under
+``torch.utils.checkpoint``.
+
+Complex gradients follow PyTorch's convention: for a real loss L the gradient
+of a complex tensor z is dL/dRe z + i dL/dIm z.
+"""
+
+from __future__ import annotations
+
+import math
+
+import torch
+import torch.utils.checkpoint as ckpt
+from torch.autograd import Function
+
+from . import _lib, ops
+from .dtypes import complex_dtype
+from .fourier import _fftn, _ifftn, hip_nufft
+
+
+class _Ctx:
+    """Everything one differentiable SENSE application needs for one trajectory."""
+
+    def __init__(self, nufft: hip_nufft, trj: torch.Tensor, mps: torch.Tensor | None, dcf: torch.Tensor | None):
+        _lib.require_device(trj)
+        self.nufft = nufft
+        self.im_size = tuple(nufft.im_size)
+        self.d = len(self.im_size)
+        self.G = nufft.os_shape()
+        self.trj_shape = tuple(trj.shape[:-1])
+        self.plan = nufft.plan(trj.detach().reshape(-1, self.d))
+        self.mps = None if mps is None else mps.to(complex_dtype).contiguous()
+        self.C = 1 if mps is None else mps.shape[0]
+        self.dcf = None if dcf is None else dcf.to(torch.float32).reshape(-1).contiguous()
+        self.apod = nufft.apod_table(trj.device)
+        self.s_fwd = 1.0 / math.sqrt(math.prod(self.im_size))
+        self.s_interp = 1.0 / nufft.width ** self.d
+        self.s_adj = self.s_interp * self.s_fwd
+        sc, _ = nufft._scale_shift(self.im_size)
+        self.dt_dk = torch.tensor(sc, dtype=torch.float32, device=trj.device)
+
+    def grids(self, x: torch.Tensor) -> torch.Tensor:
+        """FFT(pad(apod * S_c x)) / sqrt(prod N): (C, *G) in the unshifted layout."""
+        x = x.to(complex_dtype).reshape(1, *self.im_size)
+        g = ops.pad_expand(x, self.im_size, self.G, smaps=self.mps, apod=self.apod, scale=self.s_fwd)
+        return _fftn(g.reshape(self.C, *self.G), self.d)
+
+    def forward(self, x):
+        out = torch.empty((self.C, self.plan.npts), dtype=complex_dtype, device=x.device)
+        return ops.interp(self.grids(x), self.plan, out_scale=self.s_interp, out=out)
+
+    def adjoint(self, y, dcf):
+        g = torch.empty((self.C, *self.G), dtype=complex_dtype, device=y.device)
+        ops.gridding(y.to(complex_dtype).reshape(self.C, -1), self.plan, g, dcf=dcf)
+        g = _ifftn(g, self.d)
+        return ops.crop_reduce(g, self.im_size, self.G, 1, smaps=self.mps, apod=self.apod, scale=self.s_adj)[0]
+
+    def dk(self, grids, a):
+        """dL/dk (M, d) from the KB-derivative gather of ``grids`` against coefficients ``a``."""
+        gt = ops.interp_grad(grids, a.to(complex_dtype).reshape(self.C, -1).contiguous(), self.plan,
+                             out_scale=self.s_interp)
+        return (gt * self.dt_dk).reshape(*self.trj_shape, self.d)
+
+
+class SenseForwardFn(Function):
+    """(x (*im), trj (*trj, d)) -> y (C, *trj): y_c = NUFFT(S_c x)."""
+
+    @staticmethod
+    def forward(ctx, x, trj, nufft, mps):
+        c = _Ctx(nufft, trj, mps, None)
+        y = c.forward(x.detach())
+        ctx.c = c
+        ctx.save_for_backward(x.detach())
+        return y.reshape(c.C, *c.trj_shape)
+
+    @staticmethod
+    def backward(ctx, gy):
+        c = ctx.c
+        (x,) = ctx.saved_tensors
+        gx = gt = None
+        gy = gy.to(complex_dtype).contiguous()
+        if ctx.needs_input_grad[0]:
+            gx = c.adjoint(gy, None).reshape(x.shape)
+            if not torch.is_complex(x):
+                gx = gx.real
+        if ctx.needs_input_grad[1]:
+            gt = c.dk(c.grids(x), gy)  # checkpointed recompute of the forward grids
+        return gx, gt, None, None
+
+
+class SenseAdjointFn(Function):
+    """(y (C, *trj), trj) -> x (*im): x = sum_c conj(S_c) NUFFT^H(dcf * y_c)."""
+
+    @staticmethod
+    def forward(ctx, y, trj, nufft, mps, dcf):
+        c = _Ctx(nufft, trj, mps, dcf)
+        x = c.adjoint(y.detach(), c.dcf)
+        ctx.c = c
+        ctx.save_for_backward(y.detach())
+        return x
+
+    @staticmethod
+    def backward(ctx, gx):
+        c = ctx.c
+        (y,) = ctx.saved_tensors
+        gy = gt = None
+        gx = gx.to(complex_dtype).contiguous()
+        if ctx.needs_input_grad[0]:
+            gy = c.forward(gx).reshape(y.shape)
+            if c.dcf is not None:
+                gy = gy * c.dcf.reshape(c.trj_shape)
+        if ctx.needs_input_grad[1]:
+            a = y.to(complex_dtype).reshape(c.C, -1)
+            if c.dcf is not None:
+                a = a * c.dcf
+            gt = c.dk(c.grids(gx), a)
+        return gy, gt, None, None, None
+
+
+def nufft_forward(img: torch.Tensor, trj: torch.Tensor, nufft: hip_nufft) -> torch.Tensor:
+    """Differentiable single-image NUFFT: img (*im_size), trj (*trj, d) -> ksp (*trj)."""
+    return SenseForwardFn.apply(img, trj, nufft, None)[0]
+
+
+def nufft_adjoint(ksp: torch.Tensor, trj: torch.Tensor, nufft: hip_nufft) -> torch.Tensor:
+    """Differentiable adjoint NUFFT: ksp (*trj), trj (*trj, d) -> img (*im_size)."""
+    return SenseAdjointFn.apply(ksp[None], trj, nufft, None, None)
+
+
+def sense_forward(x, trj, mps, nufft: hip_nufft):
+    return SenseForwardFn.apply(x, trj, nufft, mps)
+
+
+def sense_adjoint(y, trj, mps, nufft: hip_nufft, dcf=None):
+    return SenseAdjointFn.apply(y, trj, nufft, mps, dcf)
+
+
+def sense_normal(x, trj, mps, nufft: hip_nufft, dcf=None):
+    """A^H D A x, differentiable in x and trj."""
+    return sense_adjoint(sense_forward(x, trj, mps, nufft), trj, mps, nufft, dcf)
+
+
+def _cg_iter(x, r, p, rz, trj, mps, nufft, dcf, lamda, inv_eig):
+    """One CG step (algs.py:779-872 with P = identity) as differentiable torch ops."""
+    ap = sense_normal(p, trj, mps, nufft, dcf) * inv_eig + lamda * p
+    pap = torch.real(torch.sum(p.conj() * ap))
+    alpha = rz / pap
+    x = x + alpha * p
+    r = r - alpha * ap
+    rz_new = torch.real(torch.sum(r.conj() * r))
+    p = r + (rz_new / rz) * p
+    return x, r, p, rz_new
+
+
+def unrolled_cg_sense(ksp, trj, mps, nufft: hip_nufft, num_iters: int = 10, lamda_l2: float = 0.0,
+                      max_eigen: float = 1.0, dcf=None, checkpoint: bool = True):
+    """``num_iters`` CG-SENSE iterations on (A^H D A / max_eigen + lamda I) x = A^H D y / sqrt(max_eigen),
+    started at x0 = A^H D y / sqrt(max_eigen) (recons.py:77-156), differentiable in ksp, trj and mps.
+
+    With ``checkpoint`` each iteration keeps only its inputs and is recomputed
+    during backward, so memory stays at a few images per iteration.
+    """
+    inv = 1.0 / max_eigen
+    ahb = sense_adjoint(ksp, trj, mps, nufft, dcf) * (max_eigen ** -0.5)
+    x = ahb
+    if num_iters == 0:
+        return x
+    r = ahb - (sense_normal(x, trj, mps, nufft, dcf) * inv + lamda_l2 * x)
+    p = r
+    rz = torch.real(torch.sum(r.conj() * r))
+    for _ in range(num_iters):
+        if checkpoint:
+            x, r, p, rz = ckpt.checkpoint(_cg_iter, x, r, p, rz, trj, mps, nufft, dcf, lamda_l2, inv,
+                                          use_reentrant=False)
+        else:
+            x, r, p, rz = _cg_iter(x, r, p, rz, trj, mps, nufft, dcf, lamda_l2, inv)
+    return x
+
+
+__all__ = ["SenseForwardFn", "SenseAdjointFn", "nufft_forward", "nufft_adjoint", "sense_forward",
+           "sense_adjoint", "sense_normal", "unrolled_cg_sense"]

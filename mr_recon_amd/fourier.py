@@ -184,13 +184,17 @@ class hip_nufft(NUFFT):
 
     def __init__(self, im_size: tuple, oversamp: Optional[float] = 1.25, width: Optional[int] = 6,
                  beta: Optional[float] = None, apodize: Optional[bool] = True,
-                 samples_per_item: Optional[int] = None):
+                 samples_per_item=None, tune_batch: Optional[int] = None):
+        """``samples_per_item``: work-item size of the KB kernels (None: a size heuristic;
+        "auto": timed by the autotuner on the first plan of each trajectory class, for
+        ``tune_batch`` grids per call)."""
         super().__init__(im_size)
         self.oversamp = oversamp
         self.width = width
         self.apodize = apodize
         self.beta = kb_beta(width, oversamp) if beta is None else beta
         self.samples_per_item = samples_per_item
+        self.tune_batch = tune_batch
         self._plans = _PlanCache()
         self._apod_cache: dict = {}
 
@@ -248,9 +252,17 @@ class hip_nufft(NUFFT):
         t = trj_n.detach()
         if t.dtype != torch.float32:
             t = t.to(torch.float32)
-        return self._plans.get(t, (G, wrap, im_size, self.width, self.beta, self.samples_per_item),
-                               lambda: SamplePlan(t, G, self.width, self.beta, scale=sc, shift=sh, wrap=wrap,
-                                                  samples_per_item=self.samples_per_item))
+
+        def build():
+            S = self.samples_per_item
+            if S == "auto":
+                from .autotune import tune_samples_per_item
+
+                S = tune_samples_per_item(t.reshape(-1, len(G)), G, self.width, self.beta, self.tune_batch or 1,
+                                          scale=sc, shift=sh, wrap=wrap)
+            return SamplePlan(t, G, self.width, self.beta, scale=sc, shift=sh, wrap=wrap, samples_per_item=S)
+
+        return self._plans.get(t, (G, wrap, im_size, self.width, self.beta, self.samples_per_item), build)
 
     # -- split methods (reference layout) -------------------------------
     def forward_FT_only(self, img: torch.Tensor) -> torch.Tensor:
@@ -361,7 +373,7 @@ class hip_nufft(NUFFT):
         """Centred PSF kernels (N, *round(os_factor * im_size)) (fourier.py:552-582)."""
         im_size_os = tuple(round(i * os_factor) for i in self.im_size)
         nufft_os = hip_nufft(im_size=im_size_os, oversamp=self.oversamp, width=self.width,
-                             samples_per_item=self.samples_per_item)
+                             samples_per_item=self.samples_per_item, tune_batch=1)
         return calc_toep_kernel_helper(nufft_os.adjoint, trj * os_factor, weights) * (os_factor ** len(self.im_size))
 
 

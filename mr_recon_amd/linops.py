@@ -118,6 +118,8 @@ class sense_linop(linop):
 
         if self.fused:
             _lib.require_device(trj, mps)
+            if nufft.tune_batch is None:
+                nufft.tune_batch = ncoils
             self.plan = nufft.plan(trj)
             self.os_shape = nufft.os_shape()
             self.apod = nufft.apod_table(dev)

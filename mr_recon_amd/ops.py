@@ -79,8 +79,8 @@ def gridding(ksp: torch.Tensor, plan: SamplePlan, out: torch.Tensor, dcf: torch.
     st = stream_ptr(out.device)
     check(lib().mra_grid_partial(D, plan.P, ptr(ksp), ptr(dcf), ptr(plan.wsort), ptr(plan.lbase), ptr(plan.perm),
                                  ptr(g), ptr(plan.items), plan.nitems, cc, nch, ptr(scratch), st))
-    check(lib().mra_grid_reduce(D, plan.P, ptr(out), ptr(scratch), ptr(g), 0, ptr(plan.tile_items), plan.ntiles, B,
-                                int(accumulate), st))
+    check(lib().mra_grid_reduce(D, plan.P, ptr(out), ptr(scratch), ptr(g), 0, 1, ptr(plan.tile_items), plan.ntiles,
+                                B, int(accumulate), st))
     return out
 
 

@@ -35,7 +35,7 @@ def ungrid_torch(vals, locs, grid_size, width, beta, batch_size=2 ** 20, dtype=t
     """vals (B, *G) complex, locs (npts, d) already wrapped -> (B, npts) (ungrid.py:897-957)."""
     cdt = torch.complex64 if dtype == torch.float32 else torch.complex128
     vals = vals.to(cdt)
-    locs = locs.to(dtype)
+    locs = locs.to(dtype) if locs.dtype != torch.float64 else locs
     nbatch = vals.shape[0]
     ndim = locs.shape[-1]
     npts = locs.shape[0]
@@ -54,11 +54,11 @@ def ungrid_torch(vals, locs, grid_size, width, beta, batch_size=2 ** 20, dtype=t
     return out
 
 
-def grid_torch(vals, locs, grid_size, width, beta, batch_size=2 ** 20, dtype=torch.float32):
+def grid_torch(vals, locs, grid_size, width, beta, batch_size=2 ** 20, dtype=torch.float32, safe=False):
     """vals (B, npts) complex, locs (npts, d) wrapped -> (B, *G) (grid.py:556-624)."""
     cdt = torch.complex64 if dtype == torch.float32 else torch.complex128
     vals = vals.to(cdt)
-    locs = locs.to(dtype)
+    locs = locs.to(dtype) if locs.dtype != torch.float64 else locs
     ndim = locs.shape[-1]
     nbatch = vals.shape[0]
     gs = torch.tensor(grid_size)
@@ -72,7 +72,7 @@ def grid_torch(vals, locs, grid_size, width, beta, batch_size=2 ** 20, dtype=tor
     for p0 in range(0, npts, batch_size):
         p1 = min(p0 + batch_size, npts)
         gl = lower[p0:p1, None] + diff
-        w, idx, mask = _weights(locs[p0:p1], gl, radius, beta, gs)
+        w, idx, mask = _weights(locs[p0:p1], gl, radius, beta, gs, safe)
         patches = w * mask * vals[:, p0:p1, None]
         b = bidx.expand(-1, p1 - p0, patches.shape[-1])
         out.index_put_((b, *(idx[..., i] for i in range(ndim))), patches, accumulate=True)
@@ -83,27 +83,36 @@ def _width(width, ndim):
     return (float(width),) * ndim if isinstance(width, (int, float)) else tuple(float(w) for w in width)
 
 
-def ungrid(vals, locs, width, beta=1.0, dtype=torch.float32, batch_size=2 ** 20):
-    """Full ``ungrid`` surface (ungrid.py:34-63): vals (..., *G), locs (..., d) in grid units."""
+def _coord_dtype(locs, coords64):
+    return torch.float64 if coords64 else torch.float32
+
+
+def ungrid(vals, locs, width, beta=1.0, dtype=torch.float32, batch_size=2 ** 20, coords64=False, safe=False):
+    """Full ``ungrid`` surface (ungrid.py:34-63): vals (..., *G), locs (..., d) in grid units.
+
+    ``coords64`` keeps coordinates in float64 (the reference rounds them to fp32),
+    for finite-difference checks; ``safe`` masks before the square root so that
+    autograd through the weights is finite."""
     ndim = locs.shape[-1]
     grid_size = tuple(vals.shape[-ndim:])
     batch_shape = vals.shape[:-ndim]
     vflat = vals.reshape(-1, *grid_size)
-    wl = torch.remainder(locs.to(torch.float32), torch.tensor(grid_size))
+    wl = torch.remainder(locs.to(_coord_dtype(locs, coords64)), torch.tensor(grid_size))
     out = ungrid_torch(vflat, wl.reshape(-1, ndim), grid_size, _width(width, ndim), beta,
-                       batch_size=batch_size, dtype=dtype)
+                       batch_size=batch_size, dtype=dtype, safe=safe)
     return out.reshape(*batch_shape, *locs.shape[:-1])
 
 
-def grid(vals, locs, grid_size, width, beta=1.0, dtype=torch.float32, batch_size=2 ** 20):
+def grid(vals, locs, grid_size, width, beta=1.0, dtype=torch.float32, batch_size=2 ** 20, coords64=False,
+         safe=False):
     """Full ``grid`` surface (grid.py:32-67)."""
     ndim = len(grid_size)
     locs_batch = locs.shape[:-1]
     batch_shape = vals.shape[: vals.ndim - len(locs_batch)]
     vflat = vals.reshape(int(math.prod(batch_shape)), -1)
-    wl = torch.remainder(locs.to(torch.float32), torch.tensor(grid_size))
+    wl = torch.remainder(locs.to(_coord_dtype(locs, coords64)), torch.tensor(grid_size))
     out = grid_torch(vflat, wl.reshape(-1, ndim), tuple(grid_size), _width(width, ndim), beta,
-                     batch_size=batch_size, dtype=dtype)
+                     batch_size=batch_size, dtype=dtype, safe=safe)
     return out.reshape(*batch_shape, *grid_size)
 
 

@@ -66,7 +66,8 @@ class PadLastRef:
 class TritonNufftRef:
     """fourier.py:354-582 on CPU."""
 
-    def __init__(self, im_size, oversamp=1.25, width=6, beta=None, apodize=True, dtype=torch.float32):
+    def __init__(self, im_size, oversamp=1.25, width=6, beta=None, apodize=True, dtype=torch.float32,
+                 coords64=False, safe=False):
         self.im_size = tuple(im_size)
         self.oversamp = oversamp
         self.width = width
@@ -74,6 +75,10 @@ class TritonNufftRef:
         self.beta = np.pi * (((width / oversamp) * (oversamp - 0.5)) ** 2 - 0.8) ** 0.5 if beta is None else beta
         self.dtype = dtype
         self.cdtype = torch.complex64 if dtype == torch.float32 else torch.complex128
+        # coords64: scale and wrap trajectories in float64 instead of the reference's
+        # fp32 (finite differences need it); safe: autograd-safe KB masking.
+        self.coords64 = coords64
+        self.safe = safe
 
     def _apodization_func(self, x):
         apod = (self.beta ** 2 - (np.pi * self.width * x) ** 2) ** 0.5
@@ -88,7 +93,7 @@ class TritonNufftRef:
         return img
 
     def _scale_trj(self, trj, shape):
-        out = trj.clone().to(torch.float32)
+        out = trj.clone().to(torch.float64 if self.coords64 else torch.float32)
         nd = trj.shape[-1]
         for i in range(-nd, 0):
             out[..., i] *= ceil(self.oversamp * shape[i]) / shape[i]
@@ -110,10 +115,11 @@ class TritonNufftRef:
         img_shape = (*ksp_os.shape[:-nd], *self.im_size)
         N = trj.shape[0]
         t = self._scale_trj(trj, img_shape)
-        out = torch.zeros((N, *img_shape[1:-nd], *t.shape[1:-1]), dtype=self.cdtype)
+        outs = []
         for i in range(N):
-            out[i] = interp_ref.ungrid(ksp_os[i], t[i], (self.width,) * nd, beta=self.beta, dtype=self.dtype)
-        return out / self.width ** nd
+            outs.append(interp_ref.ungrid(ksp_os[i], t[i], (self.width,) * nd, beta=self.beta, dtype=self.dtype,
+                                       coords64=self.coords64, safe=self.safe))
+        return torch.stack(outs, 0) / self.width ** nd
 
     def adjoint_iFT_only(self, ksp_os):
         nd = len(self.im_size)
@@ -132,11 +138,11 @@ class TritonNufftRef:
         oshape = (N, *ksp.shape[1:-(trj.ndim - 2)], *self.im_size)
         os_shape = list(oshape)[:-nd] + [ceil(self.oversamp * i) for i in oshape[-nd:]]
         t = self._scale_trj(trj, oshape)
-        out = torch.zeros(os_shape, dtype=self.cdtype)
+        outs = []
         for i in range(N):
-            out[i] = interp_ref.grid(ksp[i], t[i], tuple(os_shape[-nd:]), (self.width,) * nd, beta=self.beta,
-                                     dtype=self.dtype)
-        return out / self.width ** nd
+            outs.append(interp_ref.grid(ksp[i], t[i], tuple(os_shape[-nd:]), (self.width,) * nd, beta=self.beta,
+                                     dtype=self.dtype, coords64=self.coords64, safe=self.safe))
+        return torch.stack(outs, 0).reshape(os_shape) / self.width ** nd
 
     def forward(self, img, trj):
         return self.forward_interp_only(self.forward_FT_only(img), trj)

@@ -1,0 +1,161 @@
+"""GPU parity of the differentiable NUFFT / SENSE: gradients with respect to the image and the
+trajectory vs (a) FP64 central finite differences of the CPU oracle and (b) autograd through the
+oracle's pure-torch path (restated with NaN-safe masking before the square root)."""
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import rel_l2
+from mr_recon_amd import synth
+from oracle.nufft_ref import TritonNufftRef
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(seed=0, im=(32, 32), C=4, nint=8, nsamp=64):
+    trj = synth.spiral_2d(nint, nsamp, im[0]).reshape(-1, 2)
+    g = torch.Generator().manual_seed(seed)
+    trj = trj + 0.3 * (torch.rand(trj.shape, generator=g) - 0.5)  # break symmetries
+    mps = synth.coil_maps(C, im, seed=seed)
+    x = synth.random_image(im, seed=seed + 1)
+    y = synth.random_image((C, trj.shape[0]), seed=seed + 2) * 0.1
+    return trj, mps, x, y
+
+
+def _oracle_loss(x, trj, mps, y, os_, w):
+    """L = sum |A(k) x - y|^2 with the reference algorithm in FP64 (float64 coordinates)."""
+    nf = TritonNufftRef(x.shape, oversamp=os_, width=w, dtype=torch.float64, coords64=True, safe=True)
+    ax = torch.stack([nf.forward((mps[c] * x)[None], trj[None])[0] for c in range(mps.shape[0])])
+    return torch.sum(torch.abs(ax - y) ** 2)
+
+
+@pytest.mark.parametrize("os_,w", [(1.25, 6), (2.0, 4)])
+def test_forward_gradients_vs_oracle_autograd(dev, os_, w):
+    from mr_recon_amd.diff import sense_forward
+    from mr_recon_amd.fourier import hip_nufft
+
+    trj, mps, x, y = _problem()
+    nf = hip_nufft(x.shape, oversamp=os_, width=w)
+    xd = x.to(dev).requires_grad_(True)
+    kd = trj.to(dev).requires_grad_(True)
+    loss = torch.sum(torch.abs(sense_forward(xd, kd, mps.to(dev), nf) - y.to(dev)) ** 2)
+    loss.backward()
+
+    x64 = x.to(torch.complex128).requires_grad_(True)
+    k64 = trj.to(torch.float64).requires_grad_(True)
+    l64 = _oracle_loss(x64, k64, mps.to(torch.complex128), y.to(torch.complex128), os_, w)
+    l64.backward()
+    assert abs(float(loss) - float(l64)) / float(l64) < 1e-4
+    assert rel_l2(xd.grad, x64.grad) < 1e-4
+    assert rel_l2(kd.grad.to(torch.float64), k64.grad) < 1e-3
+
+
+def test_trajectory_gradient_vs_fp64_finite_differences(dev):
+    from mr_recon_amd.diff import sense_forward
+    from mr_recon_amd.fourier import hip_nufft
+
+    trj, mps, x, y = _problem(seed=3)
+    nf = hip_nufft(x.shape, oversamp=1.25, width=6)
+    kd = trj.to(dev).requires_grad_(True)
+    loss = torch.sum(torch.abs(sense_forward(x.to(dev), kd, mps.to(dev), nf) - y.to(dev)) ** 2)
+    loss.backward()
+    gk = kd.grad.cpu().to(torch.float64)
+    rng = np.random.default_rng(0)
+    h = 1e-4
+    x64, m64, y64 = x.to(torch.complex128), mps.to(torch.complex128), y.to(torch.complex128)
+    for _ in range(3):
+        v = torch.from_numpy(rng.standard_normal(trj.shape))
+        k64 = trj.to(torch.float64)
+        lp = float(_oracle_loss(x64, k64 + h * v, m64, y64, 1.25, 6))
+        lm = float(_oracle_loss(x64, k64 - h * v, m64, y64, 1.25, 6))
+        fd = (lp - lm) / (2 * h)
+        an = float(torch.sum(gk * v))
+        assert abs(an - fd) / abs(fd) < 2e-3, (an, fd)
+
+
+def test_adjoint_gradients_vs_oracle_autograd(dev):
+    from mr_recon_amd.diff import sense_adjoint
+    from mr_recon_amd.fourier import hip_nufft
+
+    trj, mps, x, y = _problem(seed=5)
+    dcf = torch.rand(trj.shape[0], generator=torch.Generator().manual_seed(1)) + 0.5
+    target = synth.random_image(x.shape, seed=9)
+    nf = hip_nufft(x.shape, oversamp=1.25, width=6)
+    yd = y.to(dev).requires_grad_(True)
+    kd = trj.to(dev).requires_grad_(True)
+    out = sense_adjoint(yd, kd, mps.to(dev), nf, dcf=dcf.to(dev))
+    loss = torch.sum(torch.abs(out - target.to(dev)) ** 2)
+    loss.backward()
+
+    ref = TritonNufftRef(x.shape, oversamp=1.25, width=6, dtype=torch.float64, coords64=True, safe=True)
+    y64 = y.to(torch.complex128).requires_grad_(True)
+    k64 = trj.to(torch.float64).requires_grad_(True)
+    m64 = mps.to(torch.complex128)
+    img = sum(ref.adjoint((y64[c] * dcf)[None, None], k64[None])[0, 0] * m64[c].conj() for c in range(4))
+    l64 = torch.sum(torch.abs(img - target.to(torch.complex128)) ** 2)
+    l64.backward()
+    assert rel_l2(yd.grad, y64.grad) < 1e-4
+    assert rel_l2(kd.grad.to(torch.float64), k64.grad) < 1e-3
+
+
+def test_unrolled_cg_gradients_checkpointed(dev):
+    """Unrolled 5-iteration CG-SENSE: checkpointed and plain backward agree, and the trajectory
+    gradient matches FP64 central differences of the same unrolled recon on the oracle."""
+    from mr_recon_amd.diff import unrolled_cg_sense
+    from mr_recon_amd.fourier import hip_nufft
+
+    trj, mps, x, _ = _problem(seed=7, im=(24, 24), C=3, nint=6, nsamp=60)
+    nf = hip_nufft(x.shape, oversamp=1.25, width=6)
+    md = mps.to(dev)
+    with torch.no_grad():
+        from mr_recon_amd.diff import sense_forward
+
+        y = sense_forward(x.to(dev), trj.to(dev), md, nf)
+    target = x.to(dev)
+
+    def run(ck):
+        kd = trj.to(dev).requires_grad_(True)
+        rec = unrolled_cg_sense(y, kd, md, nf, num_iters=5, lamda_l2=1e-2, max_eigen=2.0, checkpoint=ck)
+        loss = torch.sum(torch.abs(rec - target) ** 2)
+        loss.backward()
+        return float(loss), kd.grad.clone()
+
+    l1, g1 = run(True)
+    l2, g2 = run(False)
+    assert abs(l1 - l2) <= 1e-6 * abs(l2) and rel_l2(g1, g2) < 1e-5
+
+    def oracle_unrolled(k64):
+        ref = TritonNufftRef(x.shape, oversamp=1.25, width=6, dtype=torch.float64, coords64=True)
+        m64 = mps.to(torch.complex128)
+        y64 = y.cpu().to(torch.complex128)
+
+        def fwd(v):
+            return torch.stack([ref.forward((m64[c] * v)[None], k64[None])[0] for c in range(3)])
+
+        def adj(w):
+            return sum(ref.adjoint(w[c][None, None], k64[None])[0, 0] * m64[c].conj() for c in range(3))
+
+        inv = 0.5
+        ahb = adj(y64) * 2.0 ** -0.5
+        xx = ahb
+        r = ahb - (adj(fwd(xx)) * inv + 1e-2 * xx)
+        p = r
+        rz = torch.real(torch.sum(r.conj() * r))
+        for _ in range(5):
+            ap = adj(fwd(p)) * inv + 1e-2 * p
+            a = rz / torch.real(torch.sum(p.conj() * ap))
+            xx = xx + a * p
+            r = r - a * ap
+            rzn = torch.real(torch.sum(r.conj() * r))
+            p = r + (rzn / rz) * p
+            rz = rzn
+        return float(torch.sum(torch.abs(xx - x.to(torch.complex128)) ** 2))
+
+    assert abs(oracle_unrolled(trj.to(torch.float64)) - l1) / l1 < 1e-3
+    v = torch.from_numpy(np.random.default_rng(2).standard_normal(trj.shape))
+    h = 1e-4
+    k64 = trj.to(torch.float64)
+    fd = (oracle_unrolled(k64 + h * v) - oracle_unrolled(k64 - h * v)) / (2 * h)
+    an = float(torch.sum(g1.cpu().to(torch.float64) * v))
+    assert abs(an - fd) / abs(fd) < 5e-3, (an, fd)
