@@ -62,10 +62,12 @@ int mra_interp(int ndim, int P, const void* grid, void* out, const float* wsort,
                const int* perm, const void* geoms, const void* items, int nitems, int ccoils, int nchunks,
                void* stream);
 /* mra_grid_partial + mra_grid_reduce replace grid (grid.py:32-128, Triton kernels
- * _grid1d/2d/3d grid.py:150-502, which scatter with fp32 atomics); the reduce
- * writes every grid point, so the output needs no zeroing. */
+ * _grid1d/2d/3d grid.py:150-502, which scatter with fp32 atomics). No atomics
+ * are used: phase 1 accumulates per tile in LDS with plain read-modify-writes and
+ * stores subgrids to scratch; the reduce writes every grid point once, so the output
+ * needs no zeroing. */
 int mra_grid_partial(int ndim, int P, const void* ksp, const float* dcf, const float* wsort, const int* lbase,
-                     const int* perm, const void* geoms, const void* items, int nitems, int ccoils, int nchunks,
+                     const int* perm, const void* geoms, const void* items, int nitems, int nbatch,
                      void* scratch, void* stream);
 int mra_grid_reduce(int ndim, int P, void* grid, const void* scratch, const void* geoms, int prob0, int nprob,
                     const int* tile_items, int ntiles, int nbatch, int accumulate, void* stream);

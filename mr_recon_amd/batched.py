@@ -128,14 +128,14 @@ class _Group:
         g = torch.empty((self.ncoil, *self.G), dtype=complex_dtype, device=self.device)
         st = stream_ptr(self.device)
         check(lib().mra_grid_partial(2, self.P, ptr(Y), ptr(self.dcf), ptr(self.wsort), ptr(self.lbase),
-                                     ptr(self.perm), ptr(self.geoms), ptr(self.items), self.nitems, self.cc,
-                                     self.nchunks, ptr(self.scratch), st))
+                                     ptr(self.perm), ptr(self.geoms), ptr(self.items), self.nitems, self.maxC,
+                                     ptr(self.scratch), st))
         check(lib().mra_grid_reduce(2, self.P, ptr(g), ptr(self.scratch), ptr(self.geoms), 0, self.nimg,
                                     ptr(self.tile_items), self.ntiles, self.maxC, 0, st))
         g = _ifftn(g, 2)
         X = torch.empty((self.nimg, *self.im), dtype=complex_dtype, device=self.device)
         check(lib().mra_crop_reduce_batched(2, _lib.ints(self.im), _lib.ints(self.G), None, ptr(g), self.nimg,
-                                            ptr(self.mps), ptr(self.coil_off), ptr(self.apod), self.s_adj * scale,
+                                            ptr(self.mps), ptr(self.coil_off), ptr(self.apod), self.s_fwd * scale,
                                             ptr(X), st))
         return X
 

@@ -69,8 +69,6 @@ def gridding(ksp: torch.Tensor, plan: SamplePlan, out: torch.Tensor, dcf: torch.
             raise ValueError("dcf must be float32 with one entry per sample")
         dcf = dcf.contiguous()
     g = plan.geom(B, plan.npts, out_scale)
-    cc = plan.coil_chunk(B)
-    nch = -(-B // cc)
     need = gridding_scratch_elems(plan, B)
     if scratch is None:
         scratch = torch.empty(need, dtype=torch.complex64, device=out.device)
@@ -78,7 +76,7 @@ def gridding(ksp: torch.Tensor, plan: SamplePlan, out: torch.Tensor, dcf: torch.
         raise ValueError(f"gridding scratch must be contiguous complex64 with >= {need} elements")
     st = stream_ptr(out.device)
     check(lib().mra_grid_partial(D, plan.P, ptr(ksp), ptr(dcf), ptr(plan.wsort), ptr(plan.lbase), ptr(plan.perm),
-                                 ptr(g), ptr(plan.items), plan.nitems, cc, nch, ptr(scratch), st))
+                                 ptr(g), ptr(plan.items), plan.nitems, B, ptr(scratch), st))
     check(lib().mra_grid_reduce(D, plan.P, ptr(out), ptr(scratch), ptr(g), 0, 1, ptr(plan.tile_items), plan.ntiles,
                                 B, int(accumulate), st))
     return out

@@ -84,4 +84,4 @@ def test_batched_cg_graph_replays(dev):
         eager = B.cg_sense(ks, **kw)
         _ = sum(float(e.abs().sum()) for e in eager)  # eager work + host sync between replays
         for a, b in zip(got, eager):
-            assert rel_l2(a, b) < 1e-6
+            assert rel_l2(a, b) < 1e-5

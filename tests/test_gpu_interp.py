@@ -214,7 +214,7 @@ def test_dense_hot_tile_split(dev):
     from mr_recon_amd.hip_interp import grid, ungrid
 
     g = torch.Generator().manual_seed(11)
-    k = 30 + torch.rand((20000, 2), generator=g) * 3  # all inside one 32-tile
+    k = 30 + torch.rand((20000, 2), generator=g) * 3  # all inside one tile
     vals = torch.randn((3, 64, 64), dtype=torch.complex64, generator=g)
     out = ungrid(vals.to(dev), k.to(dev), 6.0, kernel_params={"beta": 9.0})
     assert rel_l2(out, interp_ref.ungrid(vals, k, 6.0, beta=9.0)) < TOL

@@ -40,7 +40,7 @@ def test_query_functions_without_gpu():
 
     L = _lib.lib()
     assert L.mra_version() >= 100
-    assert L.mra_default_tile(3) == 8 and L.mra_default_tile(2) == 32
+    assert L.mra_default_tile(3) == 8 and L.mra_default_tile(2) == 16
     assert L.mra_subgrid_points(3, 5) == 12 ** 3
     assert L.mra_max_coil_chunk(3, 5) * 12 ** 3 * 8 <= 64 * 1024
     assert L.mra_geom_size() % 8 == 0
