@@ -40,16 +40,18 @@ int mra_geom_fill(void* dst_host, long long grid_off, long long grid_stride, lon
  *   triton_nufft._scale_trj          fourier.py:420-430   t = fl32(fl32(k*fl32(G/N)) + G//2)
  *   prep_ungrid_shapes / prep_grid   ungrid.py:851-894, grid.py:508-553   t = remainder(t, G)
  *   get_neighborhood                 triton_interp/kernels.py:531-536     lower = ceil(t - w/2)
- * bit for bit, then sorts samples (stably) by the tile of their patch's lower corner. */
+ * bit for bit, then sorts samples stably by (tile of their patch's lower corner, position
+ * of that corner in the tile) with LSD radix passes. The work area holds
+ * mra_bin_workspace(...) 4-byte words; tile_start gets ntiles + 2 entries (tile ntiles
+ * collects non-finite samples, the last entry is npts). */
 long long mra_scan_workspace(long long n);
 int mra_scan_excl_i32(const int* in, long long n, int* out, int* ws, void* stream);
-int mra_bin_sizes(long long npts, int ntiles, int chunk_len, int* nchunks, long long* hist_elems,
-                  long long* scan_ws_elems);
+long long mra_bin_workspace(int ndim, long long npts, long long ntiles, int tile, int chunk_len);
 int mra_bin_build(int ndim, const float* coords, long long npts, const int* G, const float* scale,
                   const float* shift, int apply_scale, float width, int tile, const int* wrap, int chunk_len,
-                  int* hist, int* scanned, int* scan_ws, float* tsort, int* perm, void* stream);
-int mra_bin_items(const int* scanned, int nchunks, int ntiles, int S, int prob, int* nitems_ws, int* item_off,
-                  int* scan_ws, void* items, int max_items, int* tile_start, void* stream);
+                  void* work, float* tsort, int* perm, int* tile_start, void* stream);
+int mra_bin_items(const int* tile_start, int ntiles, int S, int prob, int* nitems_ws, int* item_off,
+                  int* scan_ws, void* items, int max_items, void* stream);
 /* Kaiser-Bessel weights of every sorted sample (kernels.py:180-246 kaiser_bessel,
  * kernels.py:295-393 weights2d/weights3d, norm '1'). */
 int mra_plan_weights(int ndim, int P, const float* tsort, int npts, const int* G, const int* wrap, float width,

@@ -41,9 +41,9 @@ SIGNATURES = {
     "mra_plan_weights": (_i, [_i, _i, _vp, _i, _ip, _ip, _f, _f, _vp, _vp, _vp]),
     "mra_scan_workspace": (_ll, [_ll]),
     "mra_scan_excl_i32": (_i, [_vp, _ll, _vp, _vp, _vp]),
-    "mra_bin_sizes": (_i, [_ll, _i, _i, _ip, _llp, _llp]),
-    "mra_bin_build": (_i, [_i, _vp, _ll, _ip, _fp, _fp, _i, _f, _i, _ip, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "mra_bin_items": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp]),
+    "mra_bin_workspace": (_ll, [_i, _ll, _ll, _i, _i]),
+    "mra_bin_build": (_i, [_i, _vp, _ll, _ip, _fp, _fp, _i, _f, _i, _ip, _i, _vp, _vp, _vp, _vp, _vp]),
+    "mra_bin_items": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp]),
     "mra_interp": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "mra_grid_partial": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp]),
     "mra_grid_reduce": (_i, [_i, _i, _vp, _vp, _vp, _i, _i, _vp, _i, _i, _i, _vp]),

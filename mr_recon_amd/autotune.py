@@ -1,6 +1,6 @@
 """Autotuner for the work decomposition of the KB interp / gridding kernels.
 
-The tile edge T is fixed per dimensionality at compile time (3D 8, 2D 32, 1D
+The tile edge T is fixed per dimensionality at compile time (3D 8, 2D 16, 1D
 256); what varies with the trajectory is how many sorted samples a work item
 takes (``samples_per_item``): small items spread hot tiles (radial centres)
 over more blocks, large items cut the per-item subgrid loads and the gridding
@@ -50,7 +50,7 @@ def _save() -> None:
 
 
 def tune_key(ndim: int, width: float, grid_size, npts: int, nbatch: int) -> str:
-    ntiles = math.prod(-(-g // {1: 256, 2: 32, 3: 8}[ndim]) for g in grid_size)
+    ntiles = math.prod(-(-g // {1: 256, 2: 16, 3: 8}[ndim]) for g in grid_size)
     dens = max(1, npts // max(1, ntiles))
     return (f"d{ndim}-p{kernel_points(width)}-g{'x'.join(map(str, grid_size))}-"
             f"s{1 << int(math.log2(dens))}-b{1 << int(math.log2(max(1, nbatch)))}")

@@ -53,12 +53,22 @@ __device__ __forceinline__ int pmod(int a, int m) {
   return r < 0 ? r + m : r;
 }
 
-// MI355X dispatches workgroups round-robin over its 8 XCDs. Remap the block
-// index so that consecutive work items (neighbouring tiles, which share halo
-// grid lines) land on the same XCD and therefore the same L2.
-__device__ __forceinline__ int xcd_swizzle(int bid, int nblocks) {
-  const int per = nblocks >> 3;     // nblocks is padded to a multiple of 8
-  return (bid & 7) * per + (bid >> 3);
+// MI355X dispatches workgroups round-robin over its 8 XCDs (block b runs on XCD
+// b % 8). Work items are ordered by tile, and trajectories concentrate samples in
+// a few regions (the k-space centre), so contiguous item ranges per XCD would load
+// some XCDs several times more than others. Items are dealt round-robin instead:
+// XCD x takes items x, x+8, x+16, ...; the nsub blocks of one item (coil chunks or
+// sample slices, which read the same sample data) run back to back on one XCD and
+// share its L2. The grid holds 8 * nsub * ceil(nitems / 8) blocks.
+__device__ __forceinline__ void item_of_block(int bid, int nsub, int& item, int& sub) {
+  const int k = bid >> 3;
+  const int q = k / nsub;
+  item = (bid & 7) + 8 * q;
+  sub = k - q * nsub;
+}
+
+inline unsigned item_grid(long long nitems, int nsub) {
+  return (unsigned)(8LL * nsub * ((nitems + 7) / 8));
 }
 
 // Zeroth-order modified Bessel function I0(x), x >= 0, by the polynomial

@@ -152,8 +152,8 @@ __global__ __launch_bounds__(kThreads) void interp_kernel(const float2* __restri
                                                           int nchunks) {
   using TS = TileShape<D, P, T>;
   extern __shared__ float2 lds2[];
-  const int L = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int iidx = L / nchunks, cc = L - iidx * nchunks;
+  int iidx, cc;
+  item_of_block(blockIdx.x, nchunks, iidx, cc);
   if (iidx >= nitems) return;
   const Item it = items[iidx];
   if (it.w <= 0) return;
@@ -220,8 +220,14 @@ __global__ __launch_bounds__(kThreads) void interp_kernel(const float2* __restri
 // one instruction therefore hits a distinct address, so plain LDS loads/stores
 // replace LDS float atomics (which run ~20x slower than plain LDS traffic on
 // MI355X). Consecutive samples may touch the same addresses; LDS operations of
-// one wave complete in order, so no barrier is needed. The finished subgrids are
-// stored to the item's scratch slots with coalesced plain stores.
+// one wave complete in order, so no barrier is needed between samples.
+//
+// Sample data reach the serial loop through an LDS stage of CH samples: lane l
+// gathers sample s0+l (subgrid base, D*P weights, its coils' k-space values times
+// dcf), and the next chunk's global loads are issued before the current chunk is
+// accumulated, so their latency overlaps the LDS work instead of stalling every
+// sample. The finished subgrids are stored to the item's scratch slots with
+// coalesced plain stores.
 //
 // Scratch slot of (item, batch entry): scratch + scratch_off + ((item - item_base) * nbatch + b) * SUBN
 template <int D, int P, int T>
@@ -233,7 +239,69 @@ struct RowLanes {
   static constexpr int BYLANES = 64 / LPS;
   static constexpr int BYLDS = (32 * 1024) / (SUBN * 8);
   static constexpr int NS = (BYLANES < BYLDS ? BYLANES : BYLDS) < 1 ? 1 : (BYLANES < BYLDS ? BYLANES : BYLDS);
+  // LDS stage of CH samples: rec[CH][RECW] = {base, dim-0 weights}, the weights of dims
+  // 1..D-1 as [(D-1)*P][WS] and the k-space values as float2 [NS][WS]. WS = 65 puts the
+  // P weights one lane may pick for a sample in distinct banks. Coil slots of the
+  // subgrid are SSTR apart; the odd padding spreads the slots over different banks.
+  static constexpr int CH = 32;                                   // samples per stage
+  static constexpr int RECW = (P + 1 + 3) / 4 * 4;
+  static constexpr int WS = CH + 1;
+  static constexpr int NWL = (D - 1) * P > 0 ? (D - 1) * P : 1;
+  static constexpr int SSTR = NS > 1 ? SUBN + 6 : SUBN;
+  static constexpr int LDS_FLOAT2 = (NS - 1) * SSTR + SUBN;
 };
+
+template <int D, int P, int NS>
+struct StageRegs {
+  int base;
+  float w[D * P];
+  float2 v[NS];
+};
+
+template <int D, int P, int T>
+__device__ __forceinline__ void stage_load(StageRegs<D, P, RowLanes<D, P, T>::NS>& r, bool ok, long long gs, int o,
+                                           const float2* __restrict__ ksp, const float* __restrict__ dcf,
+                                           const float* __restrict__ wsort, const int* __restrict__ lbase,
+                                           const Geom& g, int c0, int nc) {
+  using RL = RowLanes<D, P, T>;
+  r.base = ok ? lbase[gs] : 0;
+  const float* w = wsort + (ok ? gs : 0) * (D * P);
+#pragma unroll
+  for (int e = 0; e < D * P; ++e) r.w[e] = ok ? w[e] : 0.0f;
+  const float sc = (ok && dcf) ? dcf[g.samp_off + o] * g.out_scale : g.out_scale;
+  const float2* kc = ksp + g.ksp_off + (long long)c0 * g.ksp_stride + (ok ? o : 0);
+#pragma unroll
+  for (int cs = 0; cs < RL::NS; ++cs) {
+    float2 v = (ok && cs < nc) ? kc[(long long)cs * g.ksp_stride] : make_float2(0.f, 0.f);
+    r.v[cs] = make_float2(v.x * sc, v.y * sc);
+  }
+}
+
+template <int D, int P, int NR>
+struct SampleView {
+  float wx[P];
+  float wr[NR];
+  float2 v;
+};
+
+template <int D, int P, int T>
+__device__ __forceinline__ void read_view(SampleView<D, P, RowLanes<D, P, T>::NR>& sv, int t, const float* rec,
+                                          const float* wl, const float2* vmine,
+                                          const int (&ridx)[RowLanes<D, P, T>::NR][D > 1 ? D - 1 : 1],
+                                          const bool (&rok)[RowLanes<D, P, T>::NR]) {
+  using RL = RowLanes<D, P, T>;
+  const float* rt = rec + t * RL::RECW;
+#pragma unroll
+  for (int i = 0; i < P; ++i) sv.wx[i] = rt[1 + i];
+  sv.v = vmine[t];
+#pragma unroll
+  for (int q = 0; q < RL::NR; ++q) {
+    float wr = 1.0f;
+    if constexpr (D == 3) wr = wl[ridx[q][0] * RL::WS + t] * wl[(P + ridx[q][1]) * RL::WS + t];
+    else if constexpr (D == 2) wr = wl[ridx[q][0] * RL::WS + t];
+    sv.wr[q] = rok[q] ? wr : 0.0f;
+  }
+}
 
 template <int D, int P, int T>
 __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restrict__ ksp,
@@ -248,8 +316,11 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
   using RL = RowLanes<D, P, T>;
   constexpr int SUB = T + P - 1;
   extern __shared__ float2 lds2[];
-  const int L = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int iidx = L / nchunks, cc = L - iidx * nchunks;
+  __shared__ float rec[RL::CH * RL::RECW];
+  __shared__ float wl[RL::NWL * RL::WS];
+  __shared__ float2 vst[RL::NS * RL::WS];
+  int iidx, cc;
+  item_of_block(blockIdx.x, nchunks, iidx, cc);
   if (iidx >= nitems) return;
   const Item it = items[iidx];
   if (it.w <= 0 || it.y < 0) return;  // non-finite samples contribute nothing
@@ -258,22 +329,32 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
   if (c0 >= g.nbatch) return;
   const int nc = min(RL::NS, g.nbatch - c0);
   const int lane = threadIdx.x;
+  const int cnt = it.w;
+  const long long gs0 = g.samp_off + it.z;
 
-  for (int e = lane; e < nc * TS::SUBN; e += 64) lds2[e] = make_float2(0.f, 0.f);
-  __syncthreads();
+  // first chunk's gathers go out before the subgrid is cleared
+  StageRegs<D, P, RL::NS> sr;
+  constexpr int CH = RL::CH;
+  {
+    const bool ok = lane < CH && lane < cnt;
+    const int o = ok ? perm[gs0 + lane] : 0;
+    stage_load<D, P, T>(sr, ok, gs0 + lane, o, ksp, dcf, wsort, lbase, g, c0, nc);
+  }
+  int onext = (lane < CH && CH + lane < cnt) ? perm[gs0 + CH + lane] : 0;
+
+  for (int e = lane; e < (nc - 1) * RL::SSTR + TS::SUBN; e += 64) lds2[e] = make_float2(0.f, 0.f);
 
   // this lane's coil slot and rest positions
   const int cs = lane / RL::LPS;
   const int rl = lane - cs * RL::LPS;
   const bool active = cs < nc;
-  int rpos[RL::NR], roff[RL::NR], ridx[RL::NR][D > 1 ? D - 1 : 1];
+  int roff[RL::NR], ridx[RL::NR][D > 1 ? D - 1 : 1];
   bool rok[RL::NR];
 #pragma unroll
   for (int q = 0; q < RL::NR; ++q) {
     const int r = rl + q * 64;
     rok[q] = active && r < RL::PR;
     const int rr = rok[q] ? r : 0;
-    rpos[q] = rr;
     if constexpr (D == 3) {
       ridx[q][0] = rr / P;
       ridx[q][1] = rr % P;
@@ -286,41 +367,91 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
       roff[q] = 0;
     }
   }
-  float2* sgrid = lds2 + cs * TS::SUBN;
-  const float2* kcol = ksp + g.ksp_off + (long long)(c0 + (active ? cs : 0)) * g.ksp_stride;
+  float2* sgrid = lds2 + (active ? cs : 0) * RL::SSTR;
+  const float2* vmine = vst + (active ? cs : 0) * RL::WS;
 
-  for (int s = it.z; s < it.z + it.w; ++s) {
-    const long long gs = g.samp_off + s;
-    const float* w = wsort + gs * (D * P);
-    const int base = lbase[gs];
-    const int o = perm[gs];
-    const float dk = dcf ? dcf[g.samp_off + o] : 1.0f;
-    float2 v = active ? kcol[o] : make_float2(0.f, 0.f);
-    v.x *= dk * g.out_scale;
-    v.y *= dk * g.out_scale;
-    float wx[P];
-#pragma unroll
-    for (int i = 0; i < P; ++i) wx[i] = w[i];
-#pragma unroll
-    for (int q = 0; q < RL::NR; ++q) {
-      if (!rok[q]) continue;
-      float wr = 1.0f;
-      if constexpr (D == 3) wr = w[P + ridx[q][0]] * w[2 * P + ridx[q][1]];
-      else if constexpr (D == 2) wr = w[P + ridx[q][0]];
-      const float vx = v.x * wr, vy = v.y * wr;
-      float2* pt = sgrid + base + roff[q];
-#pragma unroll
-      for (int i = 0; i < P; ++i) {
-        float2 a = pt[i * TS::SUBD1];
-        a.x += wx[i] * vx;
-        a.y += wx[i] * vy;
-        pt[i * TS::SUBD1] = a;
-      }
+  int pub = 0;  // lane l: base of stage sample l, bit 30 = last sample of its run
+  for (int s0 = 0; s0 < cnt; s0 += CH) {
+    // publish the gathered chunk
+    {
+      // run ends where the next sample's base differs, and at the end of the stage
+      const int nb = __shfl_down(sr.base, 1);
+      const bool last = lane + 1 >= CH || s0 + lane + 1 >= cnt || nb != sr.base;
+      pub = sr.base | (last ? (1 << 30) : 0);
     }
+    if (lane < CH) {
+      float* rr = rec + lane * RL::RECW;
+      rr[0] = 0.0f;
+#pragma unroll
+      for (int i = 0; i < P; ++i) rr[1 + i] = sr.w[i];
+#pragma unroll
+      for (int e = 0; e < (D - 1) * P; ++e) wl[e * RL::WS + lane] = sr.w[P + e];
+#pragma unroll
+      for (int c = 0; c < RL::NS; ++c) vst[c * RL::WS + lane] = sr.v[c];
+    }
+    __syncthreads();
+    // issue the next chunk's gathers; they land while this chunk is accumulated
+    if (s0 + CH < cnt) {
+      const int sl = s0 + CH + lane;
+      stage_load<D, P, T>(sr, lane < CH && sl < cnt, gs0 + sl, onext, ksp, dcf, wsort, lbase, g, c0, nc);
+      onext = (lane < CH && sl + CH < cnt) ? perm[gs0 + sl + CH] : 0;
+    }
+    const int n = min(CH, cnt - s0);
+    // sample t's stage values sit in registers while t+1's are read (the stage and
+    // the subgrids are distinct LDS objects, so these reads need not wait for the
+    // subgrid stores of sample t)
+    SampleView<D, P, RL::NR> cur, nxt;
+    read_view<D, P, T>(cur, 0, rec, wl, vmine, ridx, rok);
+    // Samples of a tile are ordered by patch corner, so runs share one base: their
+    // contributions are summed in registers and each run costs one read-modify-write.
+    float2 acc[RL::NR][P];
+#pragma unroll
+    for (int q = 0; q < RL::NR; ++q)
+#pragma unroll
+      for (int i = 0; i < P; ++i) acc[q][i] = make_float2(0.f, 0.f);
+    for (int t = 0; t < n; ++t) {
+      read_view<D, P, T>(nxt, t + 1 < n ? t + 1 : t, rec, wl, vmine, ridx, rok);
+      const int raw = __builtin_amdgcn_readlane(pub, t);  // scalar, no LDS round trip
+      const int cbase = raw & 0x3fffffff;
+#pragma unroll
+      for (int q = 0; q < RL::NR; ++q) {
+        const float vx = cur.v.x * cur.wr[q], vy = cur.v.y * cur.wr[q];
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+          acc[q][i].x += cur.wx[i] * vx;
+          acc[q][i].y += cur.wx[i] * vy;
+        }
+      }
+      if (raw >> 30) {  // last sample of its run (uniform)
+#pragma unroll
+        for (int q = 0; q < RL::NR; ++q) {
+          if (rok[q]) {
+            float2* pt = sgrid + cbase + roff[q];
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+              float2 a = pt[i * TS::SUBD1];
+              a.x += acc[q][i].x;
+              a.y += acc[q][i].y;
+              pt[i * TS::SUBD1] = a;
+            }
+          }
+#pragma unroll
+          for (int i = 0; i < P; ++i) acc[q][i] = make_float2(0.f, 0.f);
+        }
+      }
+      cur = nxt;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   float2* dst = scratch + g.scratch_off + ((long long)(iidx - g.item_base) * g.nbatch + c0) * TS::SUBN;
-  for (int e = lane; e < nc * TS::SUBN; e += 64) dst[e] = lds2[e];
+  constexpr int NFULL = TS::SUBN / 64, TAIL = TS::SUBN % 64;
+  for (int c = 0; c < nc; ++c) {
+    const float2* src = lds2 + c * RL::SSTR + lane;
+    float2* d = dst + c * TS::SUBN + lane;
+#pragma unroll 8
+    for (int q = 0; q < NFULL; ++q) d[q * 64] = src[q * 64];
+    if (TAIL && lane < TAIL) d[NFULL * 64] = src[NFULL * 64];
+  }
 }
 
 // One block per (problem, output tile, batch entry): sums the covering subgrids in a
@@ -453,8 +584,8 @@ __global__ __launch_bounds__(kThreads) void interp_grad_kernel(const float2* __r
                                                                int nsub) {
   using TS = TileShape<D, P, T>;
   extern __shared__ float2 lds2[];
-  const int L = xcd_swizzle(blockIdx.x, gridDim.x);
-  const int iidx = L / nsub, sub = L - iidx * nsub;
+  int iidx, sub;
+  item_of_block(blockIdx.x, nsub, iidx, sub);
   if (iidx >= nitems) return;
   const Item it = items[iidx];
   const int s0 = it.z + sub * kThreads * K;
@@ -623,7 +754,6 @@ struct LaunchArgs {
   hipStream_t stream;
 };
 
-static unsigned pad8(long long n) { return (unsigned)((n + 7) / 8 * 8); }
 
 template <int D, int P>
 static int launch_op(const LaunchArgs& a) {
@@ -638,7 +768,7 @@ static int launch_op(const LaunchArgs& a) {
     case Op::Interp: {
       if (a.nitems <= 0) break;
       const size_t sh = (size_t)a.ccoils * TS::SUBN * sizeof(float2);
-      interp_kernel<D, P, T><<<pad8((long long)a.nitems * a.nchunks), kThreads, sh, a.stream>>>(
+      interp_kernel<D, P, T><<<item_grid(a.nitems, a.nchunks), kThreads, sh, a.stream>>>(
           (const float2*)a.src, (float2*)a.dst, a.wsort, a.lbase, a.perm, a.geoms, a.items, a.nitems, a.ccoils,
           a.nchunks);
       break;
@@ -647,8 +777,8 @@ static int launch_op(const LaunchArgs& a) {
       if (a.nitems <= 0) break;
       using RL = RowLanes<D, P, T>;
       const int nch = (a.nbatch + RL::NS - 1) / RL::NS;
-      const size_t sh = (size_t)RL::NS * TS::SUBN * sizeof(float2);
-      grid_partial_kernel<D, P, T><<<pad8((long long)a.nitems * nch), 64, sh, a.stream>>>(
+      const size_t sh = (size_t)RL::LDS_FLOAT2 * sizeof(float2);
+      grid_partial_kernel<D, P, T><<<item_grid(a.nitems, nch), 64, sh, a.stream>>>(
           (const float2*)a.src, (const float*)a.aux, a.wsort, a.lbase, a.perm, a.geoms, a.items, a.nitems, nch,
           (float2*)a.dst);
       break;
@@ -667,7 +797,7 @@ static int launch_op(const LaunchArgs& a) {
     case Op::Grad: {
       if (a.nitems <= 0) break;
       const int nsub = a.nchunks;  // sub-slices of kThreads*kGradK samples per item
-      interp_grad_kernel<D, P, T, kGradK><<<pad8((long long)a.nitems * nsub), kThreads,
+      interp_grad_kernel<D, P, T, kGradK><<<item_grid(a.nitems, nsub), kThreads,
                                             2 * TS::SUBN * sizeof(float2), a.stream>>>(
           (const float2*)a.src, (const float2*)a.aux, (float*)a.dst, a.tsort, a.perm, a.geoms, a.items, a.nitems,
           nsub);

@@ -177,7 +177,10 @@ class sense_linop(linop):
         img = torch.empty((1, *self.im_size), dtype=complex_dtype, device=self.torch_dev)
         for i, (c0, c1) in enumerate(self._coil_chunks()):
             g = torch.empty((c1 - c0, *self.os_shape), dtype=complex_dtype, device=self.torch_dev)
-            self._mark("grid", ops.gridding, y[c0:c1], self.plan, g, dcf=self.dcf.reshape(-1))
+            sc = self._mark("grid_partial", ops.gridding_partial, y[c0:c1], self.plan, c1 - c0,
+                            dcf=self.dcf.reshape(-1))
+            self._mark("grid_reduce", ops.gridding_reduce, sc, self.plan, g)
+            del sc
             g = self._mark("ifft", _ifftn, g, d)
             self._mark("crop_reduce", ops.crop_reduce, g, self.im_size, self.os_shape, 1, smaps=self.mps[c0:c1],
                        apod=self.apod, scale=self._adj_scale, out=img, accumulate=i > 0)

@@ -54,14 +54,27 @@ def gridding(ksp: torch.Tensor, plan: SamplePlan, out: torch.Tensor, dcf: torch.
     sample first. ``scratch`` (optional) holds the per-item partial subgrids,
     ``gridding_scratch_elems(plan, B)`` complex64 elements.
     """
-    _lib.require_device(ksp, out, dcf)
-    ksp = _c64(ksp, "ksp")
+    B = _grid_batch(plan, out)
+    scratch = gridding_partial(ksp, plan, B, dcf=dcf, out_scale=out_scale, scratch=scratch)
+    return gridding_reduce(scratch, plan, out, accumulate=accumulate)
+
+
+def _grid_batch(plan: SamplePlan, out: torch.Tensor) -> int:
     D = plan.ndim
     if out.dtype != torch.complex64 or not out.is_contiguous():
         raise TypeError("gridding output must be a contiguous complex64 tensor")
     if tuple(out.shape[-D:]) != plan.grid_size:
         raise ValueError(f"output grid {tuple(out.shape[-D:])} does not match plan {plan.grid_size}")
-    B = int(math.prod(out.shape[:-D]))
+    return int(math.prod(out.shape[:-D]))
+
+
+def gridding_partial(ksp: torch.Tensor, plan: SamplePlan, nbatch: int, dcf: torch.Tensor | None = None,
+                     out_scale: float = 1.0, scratch: torch.Tensor | None = None) -> torch.Tensor:
+    """Gridding phase 1: every work item's samples summed into its subgrids in
+    ``scratch`` (returned; allocated when not given)."""
+    _lib.require_device(ksp, dcf)
+    ksp = _c64(ksp, "ksp")
+    B = int(nbatch)
     if ksp.numel() != B * plan.npts:
         raise ValueError(f"gridding: ksp has {ksp.numel()} elements, expected {B}x{plan.npts}")
     if dcf is not None:
@@ -71,14 +84,25 @@ def gridding(ksp: torch.Tensor, plan: SamplePlan, out: torch.Tensor, dcf: torch.
     g = plan.geom(B, plan.npts, out_scale)
     need = gridding_scratch_elems(plan, B)
     if scratch is None:
-        scratch = torch.empty(need, dtype=torch.complex64, device=out.device)
+        scratch = torch.empty(need, dtype=torch.complex64, device=ksp.device)
     elif scratch.numel() < need or scratch.dtype != torch.complex64 or not scratch.is_contiguous():
         raise ValueError(f"gridding scratch must be contiguous complex64 with >= {need} elements")
-    st = stream_ptr(out.device)
-    check(lib().mra_grid_partial(D, plan.P, ptr(ksp), ptr(dcf), ptr(plan.wsort), ptr(plan.lbase), ptr(plan.perm),
-                                 ptr(g), ptr(plan.items), plan.nitems, B, ptr(scratch), st))
-    check(lib().mra_grid_reduce(D, plan.P, ptr(out), ptr(scratch), ptr(g), 0, 1, ptr(plan.tile_items), plan.ntiles,
-                                B, int(accumulate), st))
+    check(lib().mra_grid_partial(plan.ndim, plan.P, ptr(ksp), ptr(dcf), ptr(plan.wsort), ptr(plan.lbase),
+                                 ptr(plan.perm), ptr(g), ptr(plan.items), plan.nitems, B, ptr(scratch),
+                                 stream_ptr(ksp.device)))
+    return scratch
+
+
+def gridding_reduce(scratch: torch.Tensor, plan: SamplePlan, out: torch.Tensor,
+                    accumulate: bool = False) -> torch.Tensor:
+    """Gridding phase 2: each output tile sums the scratch subgrids covering it."""
+    _lib.require_device(scratch, out)
+    B = _grid_batch(plan, out)
+    if scratch.numel() < gridding_scratch_elems(plan, B):
+        raise ValueError("gridding scratch too small for this batch")
+    g = plan.geom(B, plan.npts, 1.0)
+    check(lib().mra_grid_reduce(plan.ndim, plan.P, ptr(out), ptr(scratch), ptr(g), 0, 1, ptr(plan.tile_items),
+                                plan.ntiles, B, int(accumulate), stream_ptr(out.device)))
     return out
 
 

@@ -2,10 +2,10 @@
 
 A plan is built once per (trajectory, grid geometry, kernel width) and reused by
 every interp / gridding / gradient call on that trajectory, so the CG loop pays
-for binning once. Building it runs three device passes (per-chunk tile
-histograms, a scan, a stable scatter) and two small ones for the work items;
-there is no host synchronisation, so plans can also be rebuilt every step when
-the trajectory itself is being optimised.
+for binning once. Building it runs a stable LSD radix sort of the samples by
+(tile, corner position in the tile) -- two or three counting-sort passes of a
+per-chunk histogram, a scan and a stable scatter -- and two small passes for the
+work items; the only host synchronisation is the work-item count.
 
 Coordinate handling is bit-exact with the reference:
   * ``triton_nufft._scale_trj`` (fourier.py:420-430): t = fl32(fl32(k * fl32(G/N)) + G//2)
@@ -99,32 +99,28 @@ class SamplePlan:
 
         flat = coords.reshape(-1, D).contiguous()
         dev = self.device
-        nchunks = C.c_int()
-        hist_elems = C.c_longlong()
-        scan_elems = C.c_longlong()
-        check(L.mra_bin_sizes(self.npts, self.ntiles, CHUNK_LEN, C.byref(nchunks), C.byref(hist_elems),
-                              C.byref(scan_elems)))
-        hist = torch.empty(hist_elems.value, dtype=torch.int32, device=dev)
-        scanned = torch.empty(hist_elems.value + 1, dtype=torch.int32, device=dev)
-        scan_ws = torch.empty(scan_elems.value, dtype=torch.int32, device=dev)
+        wwords = int(L.mra_bin_workspace(D, self.npts, self.ntiles, self.tile, CHUNK_LEN))
+        if wwords < 0:
+            check(1)
+        work = torch.empty(wwords, dtype=torch.int32, device=dev)
         self.tsort = torch.empty(self.npts * D, dtype=torch.float32, device=dev)
         self.perm = torch.empty(self.npts, dtype=torch.int32, device=dev)
+        nkeys = self.ntiles + 1
+        self.tile_start = torch.empty(nkeys + 1, dtype=torch.int32, device=dev)
         apply = scale is not None
         sc = _lib.floats(scale if apply else [1.0] * D)
         sh = _lib.floats(shift if shift is not None else [0.0] * D)
         st = stream_ptr(dev)
         check(L.mra_bin_build(D, ptr(flat), self.npts, _lib.ints(self.grid_size), sc, sh, int(apply),
-                              self.width, self.tile, _lib.ints(self.wrap), CHUNK_LEN, ptr(hist), ptr(scanned),
-                              ptr(scan_ws), ptr(self.tsort), ptr(self.perm), st))
-        nkeys = self.ntiles + 1
+                              self.width, self.tile, _lib.ints(self.wrap), CHUNK_LEN, ptr(work), ptr(self.tsort),
+                              ptr(self.perm), ptr(self.tile_start), st))
         self.max_items = nkeys + (self.npts + self.S - 1) // self.S
         self.items = torch.empty((self.max_items, 4), dtype=torch.int32, device=dev)
         nitems_ws = torch.empty(nkeys + 1, dtype=torch.int32, device=dev)
         item_off = torch.empty(nkeys + 1, dtype=torch.int32, device=dev)
         scan_ws2 = torch.empty(L.mra_scan_workspace(nkeys + 1), dtype=torch.int32, device=dev)
-        self.tile_start = torch.empty(nkeys + 1, dtype=torch.int32, device=dev)
-        check(L.mra_bin_items(ptr(scanned), nchunks.value, self.ntiles, self.S, 0, ptr(nitems_ws), ptr(item_off),
-                              ptr(scan_ws2), ptr(self.items), self.max_items, ptr(self.tile_start), st))
+        check(L.mra_bin_items(ptr(self.tile_start), self.ntiles, self.S, 0, ptr(nitems_ws), ptr(item_off),
+                              ptr(scan_ws2), ptr(self.items), self.max_items, st))
         # item_off[t] is the first work item of tile t (t = ntiles: non-finite samples)
         self.tile_items = item_off
         # KB weights (D x P per sorted sample) and subgrid offsets, once per trajectory
@@ -139,7 +135,7 @@ class SamplePlan:
         self.max_coil_chunk = int(L.mra_max_coil_chunk(D, self.P))
         # The workspaces go back to the caching allocator here; PyTorch only
         # hands them out again to work ordered after these kernels on this stream.
-        del hist, scanned, scan_ws, nitems_ws, scan_ws2, flat
+        del work, nitems_ws, scan_ws2, flat
         self._geoms: dict[GeomKey, torch.Tensor] = {}
 
     def coil_chunk(self, nbatch: int) -> int:

@@ -7,9 +7,9 @@ wrapping ``prep_ungrid_shapes`` (ungrid.py:864-865) / ``prep_grid_shapes``
 plus G when the result is non-zero and negative; and the patch corner
 ``get_neighborhood`` (kernels.py:531-536) / ``ungrid_torch`` (ungrid.py:930):
 lower = ceil(fl32(t - w/2)). The tile binning (lower corner, shifted by
-``wrap`` and wrapped mod G, divided by the tile edge; stable order within a
-tile) has no reference counterpart: it is the build's own plan and is defined
-here.
+``wrap`` and wrapped mod G, divided by the tile edge; within a tile ordered by
+the corner's position in the tile, then by sample index) has no reference
+counterpart: it is the build's own plan and is defined here.
 """
 
 from __future__ import annotations
@@ -45,18 +45,25 @@ def lower_corner(t: np.ndarray, width: float) -> np.ndarray:
         return np.ceil((t - f32(width / 2.0)).astype(f32))
 
 
-def tile_keys(t: np.ndarray, grid_size, width, tile, wrap):
-    """Linear tile id per sample; non-finite samples get ntiles."""
+def tile_keys(t: np.ndarray, grid_size, width, tile, wrap, with_local: bool = False):
+    """Linear tile id per sample (non-finite samples get ntiles) and, with
+    ``with_local``, the row-major position of the wrapped lower corner inside its
+    tile (0 for non-finite samples)."""
     ntile = [(g + tile - 1) // tile for g in grid_size]
     ntiles = math.prod(ntile)
     ok = np.all(np.isfinite(t), axis=-1)
     lo = lower_corner(t, width)
     lo = np.where(np.isfinite(lo), lo, 0).astype(np.int64)
     key = np.zeros(t.shape[0], dtype=np.int64)
+    loc = np.zeros(t.shape[0], dtype=np.int64)
     for d, G in enumerate(grid_size):
         lw = np.mod(lo[:, d] + wrap[d], G)
         key = key * ntile[d] + lw // tile
-    return np.where(ok, key, ntiles), ntiles
+        loc = loc * tile + lw % tile
+    key = np.where(ok, key, ntiles)
+    if with_local:
+        return key, np.where(ok, loc, 0), ntiles
+    return key, ntiles
 
 
 def bin_plan(coords, grid_size, width, tile, wrap=None, scale=None, shift=None):
@@ -66,8 +73,8 @@ def bin_plan(coords, grid_size, width, tile, wrap=None, scale=None, shift=None):
     t = scale_coords(k, scale, shift) if scale is not None else k
     t = wrap_coords(t, grid_size)
     wrap = wrap if wrap is not None else (0,) * D
-    key, ntiles = tile_keys(t, grid_size, width, tile, wrap)
-    perm = np.argsort(key, kind="stable")
+    key, loc, ntiles = tile_keys(t, grid_size, width, tile, wrap, with_local=True)
+    perm = np.lexsort((np.arange(key.size), loc, key))  # by tile, corner position, sample index
     counts = np.bincount(key, minlength=ntiles + 1)
     start = np.concatenate([[0], np.cumsum(counts)])
     return t[perm].reshape(-1), perm.astype(np.int32), start.astype(np.int32), key

@@ -77,12 +77,23 @@ def test_lower_corner_vs_bruteforce():
 
 
 def test_bin_plan_is_stable_sort():
+    """The plan order equals Python's sort of (tile, corner position in tile, index)
+    tuples built sample by sample with math.ceil / % on the wrapped coordinates."""
     k = edge_coords((40, 40), 5000, seed=4)
-    ts, perm, start, key = obins.bin_plan(k, (40, 40), 4.0, 8)
+    k[7] = np.nan  # non-finite samples sort last, corner position 0
+    ts, perm, start, key = obins.bin_plan(k, (40, 40), 4.0, 8, wrap=(-20, 3))
+    t = obins.wrap_coords(k, (40, 40))
+    tuples = []
+    for i in range(k.shape[0]):
+        if not np.all(np.isfinite(t[i])):
+            tuples.append((25, 0, i))
+            continue
+        lw = [(math.ceil(np.float32(t[i, d] - np.float32(2.0))) + (-20, 3)[d]) % 40 for d in range(2)]
+        tuples.append(((lw[0] // 8) * 5 + lw[1] // 8, (lw[0] % 8) * 8 + lw[1] % 8, i))
+    want = [x[2] for x in sorted(tuples)]
+    assert perm.tolist() == want
     assert np.all(np.diff(key[perm]) >= 0)
-    for b in range(len(start) - 1):
-        seg = perm[start[b]:start[b + 1]]
-        assert np.all(np.diff(seg) > 0)
+    assert start[-1] == k.shape[0] and start[-2] == k.shape[0] - 1
 
 
 @pytest.mark.parametrize("D,G,w", [(1, 7, 4.0), (2, 6, 4.0), (2, 9, 6.0), (3, 5, 4.0), (2, 8, 3.5)])
