@@ -17,14 +17,17 @@
 //  interp        the block stages its coils' subgrids in LDS, then each thread
 //                walks samples: 25 weight products, P^D LDS reads at
 //                compile-time offsets, one output per coil.
-//  grid_partial  the block accumulates its samples into LDS subgrids with LDS
-//                float atomics and stores each subgrid to a scratch slot with
-//                plain (coalesced) stores -- no global atomics.
+//  grid_partial  one wave accumulates its samples into LDS subgrids with plain
+//                (atomic-free) read-modify-writes, summing runs of samples in
+//                register windows, and stores each subgrid to a scratch slot with
+//                plain coalesced stores -- no global atomics.
 //  grid_reduce   one block per (output tile, coil) sums, in a fixed order, the
 //                scratch subgrids of every item whose subgrid covers the tile
 //                and writes the grid once. Device-scope float atomics run at a
 //                few G/s on MI355X, so gridding never uses them.
 //  interp_grad   derivative weights, reduced over coils per sample.
+#include <type_traits>
+
 #include "common.h"
 #include "capi_util.h"
 
@@ -213,42 +216,76 @@ __global__ __launch_bounds__(kThreads) void interp_kernel(const float2* __restri
 }
 
 // --------------------------------------------------------------------------------------------
-// Gridding phase 1, atomic-free. One wave per (item, coil chunk). The wave walks
-// the item's samples one at a time; its lanes are laid out as NS coil slots of PR
-// lanes, lane (cs, r) owning coil c0+cs and patch position r of the trailing D-1
-// dims, and looping over the P positions of dim 0. Every LDS read-modify-write of
-// one instruction therefore hits a distinct address, so plain LDS loads/stores
-// replace LDS float atomics (which run ~20x slower than plain LDS traffic on
-// MI355X). Consecutive samples may touch the same addresses; LDS operations of
-// one wave complete in order, so no barrier is needed between samples.
+// Gridding phase 1, atomic-free. One wave per (item, coil chunk).
 //
-// Sample data reach the serial loop through an LDS stage of CH samples: lane l
-// gathers sample s0+l (subgrid base, D*P weights, its coils' k-space values times
-// dcf), and the next chunk's global loads are issued before the current chunk is
-// accumulated, so their latency overlaps the LDS work instead of stalling every
-// sample. The finished subgrids are stored to the item's scratch slots with
-// coalesced plain stores.
+// Lanes are laid out as NS coil slots of PR lanes; lane (cs, r) owns coil c0+cs and
+// patch position r of the trailing D-1 dims and loops over the P positions of dim 0.
+// Every LDS read-modify-write of one instruction hits a distinct address, so plain LDS
+// loads and stores replace LDS float atomics (which run ~20x slower than plain LDS
+// traffic on MI355X). Consecutive samples may touch the same addresses, which is safe
+// because one wave's LDS operations complete in order.
+//
+// Samples of a tile are sorted by patch corner, so runs of samples share one corner:
+// their contributions are summed in registers and each run costs one read-modify-write
+// of the patch. The read half of the next run's read-modify-write is issued right after
+// the current run's writes (LDS order makes it see them), so by the time that run ends
+// its old values are already in registers and the serial loop never waits on it.
+//
+// Sample data reach the serial loop through an LDS stage of CH samples: lane l gathers
+// sample s0+l (subgrid offset, D*P weights, its coils' k-space values times dcf), and
+// the next chunk's global loads are issued before the current chunk is accumulated.
+// Within a chunk a ring of LA register sets is loaded ahead of use, and each sample's
+// offset / run-end word comes from v_readlane. The finished subgrids are stored to the
+// item's scratch slots (dense layout) with coalesced plain stores.
 //
 // Scratch slot of (item, batch entry): scratch + scratch_off + ((item - item_base) * nbatch + b) * SUBN
+
+// LDS padding of the subgrid (row pitch of the last dim += E in 3D, coil-slot stride
+// += SP), from a bank model of the run read-modify-write (64 banks, 8-byte lanes).
+template <int D, int P> struct GridPad { static constexpr int E = 0, SP = 0; };
+template <> struct GridPad<1, 5> { static constexpr int E = 0, SP = 1; };
+template <> struct GridPad<1, 9> { static constexpr int E = 0, SP = 1; };
+template <> struct GridPad<2, 2> { static constexpr int E = 0, SP = 1; };
+template <> struct GridPad<2, 5> { static constexpr int E = 0, SP = 3; };
+template <> struct GridPad<2, 8> { static constexpr int E = 0, SP = 1; };
+template <> struct GridPad<2, 9> { static constexpr int E = 0, SP = 9; };
+template <> struct GridPad<3, 4> { static constexpr int E = 0, SP = 6; };
+template <> struct GridPad<3, 5> { static constexpr int E = 0, SP = 5; };
+template <> struct GridPad<3, 9> { static constexpr int E = 2, SP = 0; };
+
 template <int D, int P, int T>
 struct RowLanes {
+  static constexpr int SUB = T + P - 1;
   static constexpr int PR = ipow(P, D - 1);
   static constexpr int SUBN = TileShape<D, P, T>::SUBN;
-  static constexpr int NR = (PR + 63) / 64;                       // rest positions per lane
+  static constexpr int NR = (PR + 63) / 64;                        // rest positions per lane
   static constexpr int LPS = PR < 64 ? PR : 64;                    // lanes per coil slot
   static constexpr int BYLANES = 64 / LPS;
   static constexpr int BYLDS = (32 * 1024) / (SUBN * 8);
   static constexpr int NS = (BYLANES < BYLDS ? BYLANES : BYLDS) < 1 ? 1 : (BYLANES < BYLDS ? BYLANES : BYLDS);
-  // LDS stage of CH samples: rec[CH][RECW] = {base, dim-0 weights}, the weights of dims
-  // 1..D-1 as [(D-1)*P][WS] and the k-space values as float2 [NS][WS]. WS = 65 puts the
-  // P weights one lane may pick for a sample in distinct banks. Coil slots of the
-  // subgrid are SSTR apart; the odd padding spreads the slots over different banks.
-  static constexpr int CH = 32;                                   // samples per stage
-  static constexpr int RECW = (P + 1 + 3) / 4 * 4;
-  static constexpr int WS = CH + 1;
+  // padded LDS layout: row pitch PJ (3D), pitch PI of dim 0, slot size and stride
+  static constexpr int PJ = D == 3 ? SUB + GridPad<D, P>::E : 1;
+  static constexpr int PI = D == 3 ? SUB * PJ : (D == 2 ? SUB : 1);
+  static constexpr int SLOT = PI * SUB;
+  static constexpr int SSTR = NS > 1 ? SLOT + GridPad<D, P>::SP : SLOT;
+  static constexpr int LDS_FLOAT2 = (NS - 1) * SSTR + SLOT;
+  // LDS stage of CH samples: rec[.][RECW] = the P dim-0 weights, the weights of dims
+  // 1..D-1 as [(D-1)*P][WS] and the k-space values as float2 [NS][WS]. WS is odd so the
+  // P weights one lane may pick for a sample fall in distinct banks, and covers the
+  // LA-1 loads the ring issues past the end of a stage.
+  static constexpr int CH = 32;                                    // samples per stage
+  static constexpr int LA = 4;                                     // register ring depth
+  static constexpr int RECW = (P + 3) / 4 * 4;
+  static constexpr int WS = CH + LA + 1;
   static constexpr int NWL = (D - 1) * P > 0 ? (D - 1) * P : 1;
-  static constexpr int SSTR = NS > 1 ? SUBN + 6 : SUBN;
-  static constexpr int LDS_FLOAT2 = (NS - 1) * SSTR + SUBN;
+  static __device__ __forceinline__ int lds_index(int e) {         // dense subgrid index -> padded
+    if constexpr (D == 3 && GridPad<D, P>::E != 0) {
+      const int i = e / (SUB * SUB), r = e - i * SUB * SUB, j = r / SUB;
+      return i * PI + j * PJ + (r - j * SUB);
+    } else {
+      return e;
+    }
+  }
 };
 
 template <int D, int P, int NS>
@@ -277,29 +314,81 @@ __device__ __forceinline__ void stage_load(StageRegs<D, P, RowLanes<D, P, T>::NS
   }
 }
 
+// Stage values of one sample as loaded (no arithmetic on them until they are used,
+// so later samples' loads stay in flight while this one is accumulated).
 template <int D, int P, int NR>
 struct SampleView {
-  float wx[P];
-  float wr[NR];
+  float wx[P];           // dim-0 weights
+  float wa[NR], wb[NR];  // weights of dims 1 and 2 at this lane's rest positions
   float2 v;
 };
 
 template <int D, int P, int T>
 __device__ __forceinline__ void read_view(SampleView<D, P, RowLanes<D, P, T>::NR>& sv, int t, const float* rec,
                                           const float* wl, const float2* vmine,
-                                          const int (&ridx)[RowLanes<D, P, T>::NR][D > 1 ? D - 1 : 1],
-                                          const bool (&rok)[RowLanes<D, P, T>::NR]) {
+                                          const int (&ridx)[RowLanes<D, P, T>::NR][D > 1 ? D - 1 : 1]) {
   using RL = RowLanes<D, P, T>;
   const float* rt = rec + t * RL::RECW;
 #pragma unroll
-  for (int i = 0; i < P; ++i) sv.wx[i] = rt[1 + i];
+  for (int i = 0; i < P; ++i) sv.wx[i] = rt[i];
   sv.v = vmine[t];
 #pragma unroll
   for (int q = 0; q < RL::NR; ++q) {
+    if constexpr (D >= 2) sv.wa[q] = wl[ridx[q][0] * RL::WS + t];
+    if constexpr (D == 3) sv.wb[q] = wl[(P + ridx[q][1]) * RL::WS + t];
+  }
+}
+
+// Old subgrid values at the patch of the run starting at stage sample t.
+template <int D, int P, int T>
+__device__ __forceinline__ void read_run(int t, int pub, const bool (&rok)[RowLanes<D, P, T>::NR],
+                                         const int (&roff)[RowLanes<D, P, T>::NR], const float2* sgrid,
+                                         float2 (&pre)[RowLanes<D, P, T>::NR][P]) {
+  using RL = RowLanes<D, P, T>;
+  const int key = __builtin_amdgcn_readlane(pub, t) & 0xffff;
+#pragma unroll
+  for (int q = 0; q < RL::NR; ++q) {
+    const float2* pt = sgrid + key + (rok[q] ? roff[q] : 0);
+#pragma unroll
+    for (int i = 0; i < P; ++i) pre[q][i] = pt[i * RL::PI];
+  }
+}
+
+// Accumulate stage sample t; at the end of its run (bit 30 of its published word) write
+// old + sum back, clear the sums and start reading the next run's old values.
+template <int D, int P, int T>
+__device__ __forceinline__ void accumulate_sample(const SampleView<D, P, RowLanes<D, P, T>::NR>& sv, int t, int n,
+                                                  int pub, const bool (&rok)[RowLanes<D, P, T>::NR],
+                                                  const int (&roff)[RowLanes<D, P, T>::NR], float2* sgrid,
+                                                  float2 (&acc)[RowLanes<D, P, T>::NR][P],
+                                                  float2 (&pre)[RowLanes<D, P, T>::NR][P]) {
+  using RL = RowLanes<D, P, T>;
+  const int raw = __builtin_amdgcn_readlane(pub, t);  // scalar, no LDS round trip
+#pragma unroll
+  for (int q = 0; q < RL::NR; ++q) {
     float wr = 1.0f;
-    if constexpr (D == 3) wr = wl[ridx[q][0] * RL::WS + t] * wl[(P + ridx[q][1]) * RL::WS + t];
-    else if constexpr (D == 2) wr = wl[ridx[q][0] * RL::WS + t];
-    sv.wr[q] = rok[q] ? wr : 0.0f;
+    if constexpr (D == 3) wr = sv.wa[q] * sv.wb[q];
+    else if constexpr (D == 2) wr = sv.wa[q];
+    const float vx = sv.v.x * wr, vy = sv.v.y * wr;
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+      acc[q][i].x += sv.wx[i] * vx;
+      acc[q][i].y += sv.wx[i] * vy;
+    }
+  }
+  if (raw >> 30) {  // last sample of its run (uniform)
+    const int key = raw & 0xffff;
+#pragma unroll
+    for (int q = 0; q < RL::NR; ++q) {
+      if (rok[q]) {
+        float2* pt = sgrid + key + roff[q];
+#pragma unroll
+        for (int i = 0; i < P; ++i) pt[i * RL::PI] = make_float2(pre[q][i].x + acc[q][i].x, pre[q][i].y + acc[q][i].y);
+      }
+#pragma unroll
+      for (int i = 0; i < P; ++i) acc[q][i] = make_float2(0.f, 0.f);
+    }
+    if (t + 1 < n) read_run<D, P, T>(t + 1, pub, rok, roff, sgrid, pre);
   }
 }
 
@@ -314,9 +403,8 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
                                                           int nchunks, float2* __restrict__ scratch) {
   using TS = TileShape<D, P, T>;
   using RL = RowLanes<D, P, T>;
-  constexpr int SUB = T + P - 1;
   extern __shared__ float2 lds2[];
-  __shared__ float rec[RL::CH * RL::RECW];
+  __shared__ float rec[RL::WS * RL::RECW];
   __shared__ float wl[RL::NWL * RL::WS];
   __shared__ float2 vst[RL::NS * RL::WS];
   int iidx, cc;
@@ -331,10 +419,10 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
   const int lane = threadIdx.x;
   const int cnt = it.w;
   const long long gs0 = g.samp_off + it.z;
+  constexpr int CH = RL::CH;
 
   // first chunk's gathers go out before the subgrid is cleared
   StageRegs<D, P, RL::NS> sr;
-  constexpr int CH = RL::CH;
   {
     const bool ok = lane < CH && lane < cnt;
     const int o = ok ? perm[gs0 + lane] : 0;
@@ -342,7 +430,7 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
   }
   int onext = (lane < CH && CH + lane < cnt) ? perm[gs0 + CH + lane] : 0;
 
-  for (int e = lane; e < (nc - 1) * RL::SSTR + TS::SUBN; e += 64) lds2[e] = make_float2(0.f, 0.f);
+  for (int e = lane; e < (nc - 1) * RL::SSTR + RL::SLOT; e += 64) lds2[e] = make_float2(0.f, 0.f);
 
   // this lane's coil slot and rest positions
   const int cs = lane / RL::LPS;
@@ -358,7 +446,7 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
     if constexpr (D == 3) {
       ridx[q][0] = rr / P;
       ridx[q][1] = rr % P;
-      roff[q] = ridx[q][0] * SUB + ridx[q][1];
+      roff[q] = ridx[q][0] * RL::PJ + ridx[q][1];
     } else if constexpr (D == 2) {
       ridx[q][0] = rr;
       roff[q] = rr;
@@ -369,21 +457,26 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
   }
   float2* sgrid = lds2 + (active ? cs : 0) * RL::SSTR;
   const float2* vmine = vst + (active ? cs : 0) * RL::WS;
+  float2 acc[RL::NR][P], pre[RL::NR][P];
+#pragma unroll
+  for (int q = 0; q < RL::NR; ++q)
+#pragma unroll
+    for (int i = 0; i < P; ++i) acc[q][i] = make_float2(0.f, 0.f);
 
-  int pub = 0;  // lane l: base of stage sample l, bit 30 = last sample of its run
+  int pub = 0;  // lane l: LDS offset of stage sample l's patch, bit 30 = last of its run
   for (int s0 = 0; s0 < cnt; s0 += CH) {
-    // publish the gathered chunk
+    // publish the gathered chunk; a run ends where the next sample's corner differs,
+    // and at the end of the stage
     {
-      // run ends where the next sample's base differs, and at the end of the stage
-      const int nb = __shfl_down(sr.base, 1);
-      const bool last = lane + 1 >= CH || s0 + lane + 1 >= cnt || nb != sr.base;
-      pub = sr.base | (last ? (1 << 30) : 0);
+      const int key = RL::lds_index(sr.base);
+      const int nkey = __shfl_down(key, 1);
+      const bool last = lane + 1 >= CH || s0 + lane + 1 >= cnt || nkey != key;
+      pub = key | (last ? (1 << 30) : 0);
     }
     if (lane < CH) {
       float* rr = rec + lane * RL::RECW;
-      rr[0] = 0.0f;
 #pragma unroll
-      for (int i = 0; i < P; ++i) rr[1 + i] = sr.w[i];
+      for (int i = 0; i < P; ++i) rr[i] = sr.w[i];
 #pragma unroll
       for (int e = 0; e < (D - 1) * P; ++e) wl[e * RL::WS + lane] = sr.w[P + e];
 #pragma unroll
@@ -397,66 +490,38 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
       onext = (lane < CH && sl + CH < cnt) ? perm[gs0 + sl + CH] : 0;
     }
     const int n = min(CH, cnt - s0);
-    // sample t's stage values sit in registers while t+1's are read (the stage and
-    // the subgrids are distinct LDS objects, so these reads need not wait for the
-    // subgrid stores of sample t)
-    SampleView<D, P, RL::NR> cur, nxt;
-    read_view<D, P, T>(cur, 0, rec, wl, vmine, ridx, rok);
-    // Samples of a tile are ordered by patch corner, so runs share one base: their
-    // contributions are summed in registers and each run costs one read-modify-write.
-    float2 acc[RL::NR][P];
+    read_run<D, P, T>(0, pub, rok, roff, sgrid, pre);
+    // ring of LA register sets, loaded LA-2..LA-1 samples ahead; the stage arrays are
+    // padded so loads past the last sample stay in bounds (their values are unused)
+    SampleView<D, P, RL::NR> ring[RL::LA];
 #pragma unroll
-    for (int q = 0; q < RL::NR; ++q)
+    for (int u = 0; u < RL::LA - 1; ++u) read_view<D, P, T>(ring[u], u, rec, wl, vmine, ridx);
+    for (int t = 0; t < n; t += RL::LA) {
 #pragma unroll
-      for (int i = 0; i < P; ++i) acc[q][i] = make_float2(0.f, 0.f);
-    for (int t = 0; t < n; ++t) {
-      read_view<D, P, T>(nxt, t + 1 < n ? t + 1 : t, rec, wl, vmine, ridx, rok);
-      const int raw = __builtin_amdgcn_readlane(pub, t);  // scalar, no LDS round trip
-      const int cbase = raw & 0x3fffffff;
-#pragma unroll
-      for (int q = 0; q < RL::NR; ++q) {
-        const float vx = cur.v.x * cur.wr[q], vy = cur.v.y * cur.wr[q];
-#pragma unroll
-        for (int i = 0; i < P; ++i) {
-          acc[q][i].x += cur.wx[i] * vx;
-          acc[q][i].y += cur.wx[i] * vy;
-        }
+      for (int u = 0; u < RL::LA; ++u) {
+        if (t + u < n) accumulate_sample<D, P, T>(ring[u], t + u, n, pub, rok, roff, sgrid, acc, pre);
+        read_view<D, P, T>(ring[(u + RL::LA - 1) % RL::LA], t + u + RL::LA - 1, rec, wl, vmine, ridx);
       }
-      if (raw >> 30) {  // last sample of its run (uniform)
-#pragma unroll
-        for (int q = 0; q < RL::NR; ++q) {
-          if (rok[q]) {
-            float2* pt = sgrid + cbase + roff[q];
-#pragma unroll
-            for (int i = 0; i < P; ++i) {
-              float2 a = pt[i * TS::SUBD1];
-              a.x += acc[q][i].x;
-              a.y += acc[q][i].y;
-              pt[i * TS::SUBD1] = a;
-            }
-          }
-#pragma unroll
-          for (int i = 0; i < P; ++i) acc[q][i] = make_float2(0.f, 0.f);
-        }
-      }
-      cur = nxt;
     }
     __syncthreads();
   }
   float2* dst = scratch + g.scratch_off + ((long long)(iidx - g.item_base) * g.nbatch + c0) * TS::SUBN;
-  constexpr int NFULL = TS::SUBN / 64, TAIL = TS::SUBN % 64;
   for (int c = 0; c < nc; ++c) {
-    const float2* src = lds2 + c * RL::SSTR + lane;
-    float2* d = dst + c * TS::SUBN + lane;
+    const float2* src = lds2 + c * RL::SSTR;
+    float2* d = dst + c * TS::SUBN;
 #pragma unroll 8
-    for (int q = 0; q < NFULL; ++q) d[q * 64] = src[q * 64];
-    if (TAIL && lane < TAIL) d[NFULL * 64] = src[NFULL * 64];
+    for (int e = lane; e < TS::SUBN; e += 64) d[e] = src[RL::lds_index(e)];
   }
 }
 
-// One block per (problem, output tile, batch entry): sums the covering subgrids in a
-// fixed order. blockIdx.x = (prob - prob0) * ntiles + tile; problems of one launch
-// share the tile count.
+// Gridding phase 2. One block per (problem, output tile, group of RB batch entries):
+// sums, in a fixed order, the scratch subgrids of every item whose subgrid covers the
+// tile and writes each grid point once. blockIdx.x = (prob - prob0) * ntiles + tile,
+// blockIdx.y = batch group; problems of one launch share the tile count. A thread owns
+// NQ points of the tile for all RB batch entries, so each covering item gives it
+// NQ * RB independent loads.
+constexpr int kReduceBatch = 4;
+
 template <int D, int P, int T>
 __global__ __launch_bounds__(kThreads) void grid_reduce_kernel(float2* __restrict__ grid,
                                                                const float2* __restrict__ scratch,
@@ -464,11 +529,14 @@ __global__ __launch_bounds__(kThreads) void grid_reduce_kernel(float2* __restric
                                                                int ntiles, const int* __restrict__ tile_items,
                                                                int accumulate) {
   using TS = TileShape<D, P, T>;
+  constexpr int RB = kReduceBatch;
+  constexpr int NQ = (TS::TN + kThreads - 1) / kThreads;
   const int prob = prob0 + blockIdx.x / ntiles;
   const int tile = blockIdx.x - (prob - prob0) * ntiles;
   const Geom g = geoms[prob];
-  const int b = blockIdx.y;
-  if (b >= g.nbatch) return;
+  const int b0 = blockIdx.y * RB;
+  if (b0 >= g.nbatch) return;
+  const int nb = min(RB, g.nbatch - b0);
   int tc[D];
   tile_coords<D, T>(tile, g, tc);
   // Candidate source tiles per dim: t, t-1, t-2 (mod ntile), those whose subgrid
@@ -486,83 +554,125 @@ __global__ __launch_bounds__(kThreads) void grid_reduce_kernel(float2* __restric
       if (dist < TS::SUB || dist + T - 1 >= g.G[d]) cand[d][ncand[d]++] = tp;
     }
   }
+  // this thread's points
+  int gc[NQ][D];
+  bool inside[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int p = threadIdx.x + q * kThreads;
+    int rem = p;
+    inside[q] = p < TS::TN;
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+      gc[q][d] = tc[d] * T + rem % T;
+      rem /= T;
+      inside[q] = inside[q] && gc[q][d] < g.G[d];
+    }
+  }
+  float2 acc[NQ][RB];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+#pragma unroll
+    for (int b = 0; b < RB; ++b) acc[q][b] = make_float2(0.f, 0.f);
+
   const int* titems = tile_items + g.tile_items_off;
   const float2* sbase = scratch + g.scratch_off;
-  float2* gdst = grid + g.grid_off + (long long)b * g.grid_stride;
-  for (int p = threadIdx.x; p < TS::TN; p += kThreads) {
-    int gc[D];
-    {
-      int rem = p;
-      bool inside = true;
+  const bool simple = g.G[0] >= TS::SUB && (D < 2 || g.G[1] >= TS::SUB) && (D < 3 || g.G[D - 1] >= TS::SUB);
+  // iterate over candidate tile combinations (row-major over dims; uniform per block)
+  int k[D];
 #pragma unroll
-      for (int d = D - 1; d >= 0; --d) {
-        gc[d] = tc[d] * T + rem % T;
-        rem /= T;
-        inside = inside && gc[d] < g.G[d];
-      }
-      if (!inside) continue;
-    }
-    float ax = 0.f, ay = 0.f;
-    // iterate over candidate tile combinations (row-major over dims)
-    int k[D];
+  for (int d = 0; d < D; ++d) k[d] = 0;
+  while (true) {
+    int stile = 0;
 #pragma unroll
-    for (int d = 0; d < D; ++d) k[d] = 0;
-    while (true) {
-      bool ok = true;
-      int stile = 0;
-      int loc0[D];
+    for (int d = 0; d < D; ++d) stile = stile * g.ntile[d] + cand[d][k[d]];
+    int loc0[NQ][D];
+    bool ok[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      ok[q] = inside[q];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        const int tp = cand[d][k[d]];
-        loc0[d] = pmod(gc[d] - tp * T, g.G[d]);
-        ok = ok && loc0[d] < TS::SUB;
-        stile = stile * g.ntile[d] + tp;
+        loc0[q][d] = pmod(gc[q][d] - cand[d][k[d]] * T, g.G[d]);
+        ok[q] = ok[q] && loc0[q][d] < TS::SUB;
       }
-      if (ok) {
-        const int i0 = titems[stile], i1 = titems[stile + 1];
-        for (int it = i0; it < i1; ++it) {
-          const float2* slot = sbase + ((long long)(it - g.item_base) * g.nbatch + b) * TS::SUBN;
-          // a subgrid can wrap onto a grid point more than once when G < SUB
-          if constexpr (D == 1) {
-            for (int e0 = loc0[0]; e0 < TS::SUB; e0 += g.G[0]) {
-              const float2 v = slot[e0];
-              ax += v.x; ay += v.y;
+    }
+    const int i0 = titems[stile], i1 = titems[stile + 1];
+    for (int it = i0; it < i1; ++it) {
+      const float2* slot = sbase + ((long long)(it - g.item_base) * g.nbatch + b0) * TS::SUBN;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        if (!ok[q]) continue;
+        if (simple) {  // every subgrid point lands on a distinct grid point
+          int e = 0;
+#pragma unroll
+          for (int d = 0; d < D; ++d) e = e * TS::SUB + loc0[q][d];
+#pragma unroll
+          for (int b = 0; b < RB; ++b)
+            if (b < nb) {
+              const float2 v = slot[(long long)b * TS::SUBN + e];
+              acc[q][b].x += v.x;
+              acc[q][b].y += v.y;
             }
-          } else if constexpr (D == 2) {
-            for (int e0 = loc0[0]; e0 < TS::SUB; e0 += g.G[0])
-              for (int e1 = loc0[1]; e1 < TS::SUB; e1 += g.G[1]) {
-                const float2 v = slot[e0 * TS::SUB + e1];
-                ax += v.x; ay += v.y;
+        } else {  // G < SUB somewhere: a subgrid can wrap onto a grid point more than once
+          for (int b = 0; b < nb; ++b) {
+            const float2* sl = slot + (long long)b * TS::SUBN;
+            float ax = 0.f, ay = 0.f;
+            if constexpr (D == 1) {
+              for (int e0 = loc0[q][0]; e0 < TS::SUB; e0 += g.G[0]) {
+                ax += sl[e0].x; ay += sl[e0].y;
               }
-          } else {
-            for (int e0 = loc0[0]; e0 < TS::SUB; e0 += g.G[0])
-              for (int e1 = loc0[1]; e1 < TS::SUB; e1 += g.G[1])
-                for (int e2 = loc0[2]; e2 < TS::SUB; e2 += g.G[2]) {
-                  const float2 v = slot[(e0 * TS::SUB + e1) * TS::SUB + e2];
+            } else if constexpr (D == 2) {
+              for (int e0 = loc0[q][0]; e0 < TS::SUB; e0 += g.G[0])
+                for (int e1 = loc0[q][1]; e1 < TS::SUB; e1 += g.G[1]) {
+                  const float2 v = sl[e0 * TS::SUB + e1];
                   ax += v.x; ay += v.y;
                 }
+            } else {
+              for (int e0 = loc0[q][0]; e0 < TS::SUB; e0 += g.G[0])
+                for (int e1 = loc0[q][1]; e1 < TS::SUB; e1 += g.G[1])
+                  for (int e2 = loc0[q][2]; e2 < TS::SUB; e2 += g.G[2]) {
+                    const float2 v = sl[(e0 * TS::SUB + e1) * TS::SUB + e2];
+                    ax += v.x; ay += v.y;
+                  }
+            }
+#pragma unroll
+            for (int bb = 0; bb < RB; ++bb)
+              if (bb == b) {
+                acc[q][bb].x += ax;
+                acc[q][bb].y += ay;
+              }
           }
         }
       }
-      // next combination
-      int d = D - 1;
-      while (d >= 0) {
-        if (++k[d] < ncand[d]) break;
-        k[d] = 0;
-        --d;
-      }
-      if (d < 0) break;
     }
+    // next combination
+    int d = D - 1;
+    while (d >= 0) {
+      if (++k[d] < ncand[d]) break;
+      k[d] = 0;
+      --d;
+    }
+    if (d < 0) break;
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    if (!inside[q]) continue;
     long long lin = 0;
 #pragma unroll
-    for (int d = 0; d < D; ++d) lin = lin * g.G[d] + gc[d];
-    if (accumulate) {
-      float2 o = gdst[lin];
-      o.x += ax;
-      o.y += ay;
-      gdst[lin] = o;
-    } else {
-      gdst[lin] = make_float2(ax, ay);
+    for (int d = 0; d < D; ++d) lin = lin * g.G[d] + gc[q][d];
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      if (b >= nb) break;
+      float2* o = grid + g.grid_off + (long long)(b0 + b) * g.grid_stride + lin;
+      if (accumulate) {
+        float2 v = *o;
+        v.x += acc[q][b].x;
+        v.y += acc[q][b].y;
+        *o = v;
+      } else {
+        *o = acc[q][b];
+      }
     }
   }
 }
@@ -785,11 +895,12 @@ static int launch_op(const LaunchArgs& a) {
     }
     case Op::GridReduce: {
       if (a.ntiles <= 0 || a.nbatch <= 0 || a.nprob <= 0) break;
-      if (a.nbatch > 65535) {
-        set_last_error("grid_reduce: batch larger than 65535");
+      const int ngroups = (a.nbatch + kReduceBatch - 1) / kReduceBatch;
+      if (ngroups > 65535) {
+        set_last_error("grid_reduce: batch too large");
         return 1;
       }
-      dim3 grd((unsigned)((long long)a.ntiles * a.nprob), (unsigned)a.nbatch);
+      dim3 grd((unsigned)((long long)a.ntiles * a.nprob), (unsigned)ngroups);
       grid_reduce_kernel<D, P, T><<<grd, kThreads, 0, a.stream>>>((float2*)a.dst, (const float2*)a.src, a.geoms,
                                                                   a.prob, a.ntiles, a.tile_items, a.accumulate);
       break;

@@ -124,6 +124,51 @@ def test_grid_partial_last_tile(dev, D, G):
     assert rel_l2(y, interp_ref.ungrid(x, k, 4.0, beta=7.0)) < TOL
 
 
+@pytest.mark.parametrize("D,G", [(3, 176), (2, 1500)])
+def test_many_tiles_three_radix_passes(dev, D, G):
+    """More than 8191 tiles: the tile id is sorted in two digits (three passes in all)."""
+    from mr_recon_amd.hip_interp import grid, ungrid
+    from mr_recon_amd.plan import SamplePlan
+
+    k = _case(D, G, 4.0, 3000, seed=G)
+    grid_size = (G,) * D
+    plan = SamplePlan(k.to(dev), grid_size, 4.0, 6.0, wrap=tuple(-(g // 2) for g in grid_size))
+    assert plan.ntiles + 1 > 8192
+    ts, perm, start, _ = obins.bin_plan(k.numpy(), grid_size, 4.0, plan.tile, wrap=tuple(-(g // 2) for g in grid_size))
+    assert np.array_equal(plan.perm.cpu().numpy(), perm)
+    assert np.array_equal(plan.tsort.cpu().numpy().view(np.int32), ts.view(np.int32))
+    assert np.array_equal(plan.tile_start.cpu().numpy(), start)
+    g = torch.Generator().manual_seed(G)
+    vals = torch.randn((1, k.shape[0]), dtype=torch.complex64, generator=g)
+    out = grid(vals.to(dev), k.to(dev), grid_size, 4.0, kernel_params={"beta": 6.0})
+    assert rel_l2(out, interp_ref.grid(vals, k, grid_size, 4.0, beta=6.0)) < TOL
+    x = torch.randn((1, *grid_size), dtype=torch.complex64, generator=g)
+    y = ungrid(x.to(dev), k.to(dev), 4.0, kernel_params={"beta": 6.0})
+    assert rel_l2(y, interp_ref.ungrid(x, k, 4.0, beta=6.0)) < TOL
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_grid_runs_of_equal_corners(dev, D):
+    """Long runs of samples with one patch corner (summed in registers, across the
+    32-sample stages) mixed with isolated samples."""
+    from mr_recon_amd.hip_interp import grid
+
+    G = 40
+    g = torch.Generator().manual_seed(D)
+    pts = torch.rand((12, D), generator=g) * G
+    k = torch.cat([pts.repeat_interleave(torch.tensor([1, 2, 31, 32, 33, 64, 65, 100, 3, 1, 7, 200]), dim=0),
+                   pts[:4] + 1e-3 * torch.rand((4, D), generator=g), torch.rand((500, D), generator=g) * G])
+    k = k[torch.randperm(k.shape[0], generator=g)]
+    vals = torch.randn((3, k.shape[0]), dtype=torch.complex64, generator=g)
+    from mr_recon_amd.plan import SamplePlan
+
+    ref = interp_ref.grid(vals, k, (G,) * D, 4.0, beta=6.0)
+    for S in (64, 256, None):  # items that cut runs at different places
+        plan = SamplePlan(k.to(dev), (G,) * D, 4.0, 6.0, samples_per_item=S)
+        out = grid(vals.to(dev), k.to(dev), (G,) * D, 4.0, kernel_params={"beta": 6.0}, plan=plan)
+        assert rel_l2(out, ref) < TOL
+
+
 def test_plan_items_cover_every_sample(dev):
     """Work items: per tile contiguous, each at most S samples, together exactly the sorted range."""
     from mr_recon_amd.plan import SamplePlan
