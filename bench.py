@@ -59,6 +59,21 @@ def kernel_bytes(M):
             "grid_reduce": 8 * C * G ** 3}
 
 
+def committed_traffic(kernel):
+    """HBM bytes per launch of ``kernel`` measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes (scaled by a 1 GiB copy) and recorded in profiles/ by the profiling recipe;
+    None when no record exists for this kernel."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", f"{kernel}_3d128_c32_traffic.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    return None if m is None else {"bytes": float(m.group(1)), "source": os.path.relpath(path, ROOT)}
+
+
 def step_bytes(M):
     """Algorithmic HBM bytes of one SENSE forward+adjoint (x, maps, 7 grid passes, samples, coords)."""
     G = math.ceil(OS * N)
@@ -106,6 +121,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed steps, copy 1 GiB device-to-device once (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the NUFFT on a single MI355X "
@@ -169,6 +186,17 @@ def main():
                                               "grid_partial": "8*C*M + 4*3*M + 8*C*G^3",
                                               "grid_reduce": "8*C*G^3"}[dom]},
     }
+    traffic = committed_traffic(dom)
+    if traffic is not None:
+        rec["roofline"]["traffic"] = traffic["bytes"]
+        rec["roofline_detail"]["traffic_source"] = traffic["source"]
+    if args.copy_probe:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if not args.no_cpu_baseline:
         t0 = time.time()
         rec["cpu_baseline"] = cpu_baseline()

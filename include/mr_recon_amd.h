@@ -105,6 +105,12 @@ int mra_crop_reduce_batched(int ndim, const int* N, const int* G, const int* cen
                             void* out, void* stream);
 /* Toeplitz PSF multiply (fourier.py:344, linops.py:1285-1287). */
 int mra_cmul_bcast(void* data, const void* k, long long nbatch, long long n, void* stream);
+/* In-place unnormalised batched C2C FFT over the trailing ndim dims (rocFFT via hipFFT);
+ * replaces the centred fft / ifft of the NUFFT (fourier.py:37-61, 489-525), whose shifts
+ * are index maps of the pad / crop / KB kernels here. */
+int mra_fft_plan(int ndim, const int* n, int batch, void** plan);
+int mra_fft_exec(void* plan, void* data, int inverse, void* stream);
+int mra_fft_destroy(void* plan);
 
 /* ---- conjugate gradient -----------------------------------------------------------------
  * Vector work of conjugate_gradient (algs.py:779-872) with FP64 inner products

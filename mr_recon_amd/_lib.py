@@ -53,6 +53,9 @@ SIGNATURES = {
     "mra_pad_expand_batched": (_i, [_i, _ip, _ip, _ip, _vp, _ll, _vp, _ll, _vp, _vp, _f, _vp, _vp]),
     "mra_crop_reduce_batched": (_i, [_i, _ip, _ip, _ip, _vp, _ll, _vp, _vp, _vp, _f, _vp, _vp]),
     "mra_cmul_bcast": (_i, [_vp, _vp, _ll, _ll, _vp]),
+    "mra_fft_plan": (_i, [_i, _ip, _i, C.POINTER(_vp)]),
+    "mra_fft_exec": (_i, [_vp, _vp, _i, _vp]),
+    "mra_fft_destroy": (_i, [_vp]),
     "mra_cg_partials": (_i, []),
     "mra_cg_seg_blocks": (_i, []),
     "mra_cdot_re": (_i, [_vp, _vp, _ll, _f, _vp, _vp, _i, _vp]),

@@ -3,7 +3,8 @@
 Compiles every ``*.hip`` file in this directory with hipcc into objects under
 ``mr_recon_amd/_build/`` and links them into ``mr_recon_amd/libmr_recon_amd.so``.
 The library exports only the C-ABI declared in ``include/mr_recon_amd.h`` and
-links against nothing but the HIP runtime; PyTorch loads it with ctypes.
+links against the HIP runtime and hipFFT (rocFFT); PyTorch loads it with ctypes.
+In a PyTorch process the hipFFT already loaded by torch (same soname) is reused.
 
 Usage: ``python -m mr_recon_amd.csrc.build`` (or ``__graft_entry__.build()``).
 """
@@ -65,7 +66,9 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB.with_name(LIB.name + ".tmp")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    rocm_lib = Path(hipcc()).resolve().parent.parent / "lib"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), f"-L{rocm_lib}", "-lhipfft",
+           "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

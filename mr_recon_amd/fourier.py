@@ -19,7 +19,9 @@ the intermediate grid so they compose with code written against it.
 
 from __future__ import annotations
 
+import ctypes
 import math
+import threading
 from collections import OrderedDict
 from math import ceil
 from typing import Optional
@@ -30,6 +32,7 @@ import torch.fft as fft_torch
 import torch.nn as nn
 
 from . import _lib, ops
+from ._lib import check, lib, ptr, stream_ptr
 from .dtypes import complex_dtype, real_dtype
 from .pad import PadLast
 from .plan import SamplePlan
@@ -72,15 +75,51 @@ def calc_toep_kernel_helper(nufft_adj_os: callable, trj: torch.Tensor, weights: 
     return fft(img, dim=tuple(range(-d, 0)))
 
 
+class _FFTPlans:
+    """hipFFT plans for in-place batched C2C transforms, per (device, batch, grid shape)."""
+
+    def __init__(self):
+        self._plans: dict = {}
+        self._lock = threading.Lock()
+
+    def get(self, dev_index: int, batch: int, shape: tuple):
+        key = (dev_index, batch, shape)
+        with self._lock:
+            h = self._plans.get(key)
+            if h is None:
+                h = ctypes.c_void_p()
+                with torch.cuda.device(dev_index):
+                    check(lib().mra_fft_plan(len(shape), _lib.ints(shape), batch, ctypes.byref(h)))
+                self._plans[key] = h
+            return h
+
+
+_fft_plans = _FFTPlans()
+
+
+def _fft_inplace(x: torch.Tensor, d: int, inverse: bool) -> torch.Tensor:
+    _lib.require_device(x)
+    if x.dtype != torch.complex64 or not x.is_contiguous():
+        raise TypeError("grid FFTs run in place on contiguous complex64 tensors")
+    shape = tuple(int(s) for s in x.shape[-d:])
+    batch = int(math.prod(x.shape[:-d]))
+    if batch == 0:
+        return x
+    plan = _fft_plans.get(x.device.index, batch, shape)
+    check(lib().mra_fft_exec(plan, ptr(x), int(inverse), stream_ptr(x.device)))
+    return x
+
+
 def _fftn(x: torch.Tensor, d: int) -> torch.Tensor:
-    """Unnormalised forward FFT over the last d dims (rocFFT, out of place: torch's
-    ``out=`` form adds a full copy of the result)."""
-    return fft_torch.fftn(x, dim=tuple(range(-d, 0)))
+    """Unnormalised forward FFT over the last d dims, IN PLACE on the caller's temporary
+    grid ``x`` (returned). An out-of-place transform that preserves its input costs a
+    full copy of the grid first."""
+    return _fft_inplace(x, d, False)
 
 
 def _ifftn(x: torch.Tensor, d: int) -> torch.Tensor:
-    """Unnormalised inverse FFT (no 1/n) over the last d dims, out of place."""
-    return fft_torch.ifftn(x, dim=tuple(range(-d, 0)), norm="forward")
+    """Unnormalised inverse FFT (no 1/n) over the last d dims, in place on ``x``."""
+    return _fft_inplace(x, d, True)
 
 
 class NUFFT(nn.Module):
