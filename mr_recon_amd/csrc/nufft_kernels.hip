@@ -225,32 +225,40 @@ __global__ __launch_bounds__(kThreads) void interp_kernel(const float2* __restri
 // traffic on MI355X). Consecutive samples may touch the same addresses, which is safe
 // because one wave's LDS operations complete in order.
 //
-// Samples of a tile are sorted by patch corner, so runs of samples share one corner:
-// their contributions are summed in registers and each run costs one read-modify-write
-// of the patch. The read half of the next run's read-modify-write is issued right after
-// the current run's writes (LDS order makes it see them), so by the time that run ends
-// its old values are already in registers and the serial loop never waits on it.
+// Plane ring. The samples of an item are sorted by patch corner, dim 0 slowest, so the
+// corner's dim-0 coordinate (its plane) never decreases along the item and a sample
+// touches planes bi .. bi+P-1 only. Each coil therefore keeps a ring of P subgrid
+// planes in LDS instead of the whole subgrid: when the next run starts at a higher
+// plane, the planes below it are final -- they are stored to the item's scratch slot
+// and their ring slots cleared for the planes entering at the top. This cuts LDS per
+// coil from SUB^D to P*SUB^(D-1) points and raises the waves per CU accordingly.
 //
-// Sample data reach the serial loop through an LDS stage of CH samples: lane l gathers
-// sample s0+l (subgrid offset, D*P weights, its coils' k-space values times dcf), and
+// Runs. Samples with the same corner are summed in registers; each run costs one
+// read-modify-write of its patch, and the read half of the next run's read-modify-
+// write is issued right after the current run's writes (LDS order makes it see them),
+// so by the time that run ends its old values are already in registers.
+//
+// Stage. Sample data reach the serial loop through an LDS stage of CH samples: lane l
+// gathers sample s0+l (corner, D*P weights, its coils' k-space values times dcf), and
 // the next chunk's global loads are issued before the current chunk is accumulated.
 // Within a chunk a ring of LA register sets is loaded ahead of use, and each sample's
-// offset / run-end word comes from v_readlane. The finished subgrids are stored to the
-// item's scratch slots (dense layout) with coalesced plain stores.
+// corner / run-end word comes from v_readlane.
 //
-// Scratch slot of (item, batch entry): scratch + scratch_off + ((item - item_base) * nbatch + b) * SUBN
+// Scratch slot of (item, batch entry): scratch + scratch_off + ((item - item_base) * nbatch + b) * SUBN,
+// dense row-major subgrid.
 
-// LDS padding of the subgrid (row pitch of the last dim += E in 3D, coil-slot stride
-// += SP), from a bank model of the run read-modify-write (64 banks, 8-byte lanes).
+// LDS padding (row pitch of the last dim += E in 3D, coil-slot stride += SP), from a
+// bank model of the run read-modify-write (64 banks, 8-byte lanes) per (D, P).
 template <int D, int P> struct GridPad { static constexpr int E = 0, SP = 0; };
-template <> struct GridPad<1, 5> { static constexpr int E = 0, SP = 1; };
-template <> struct GridPad<1, 9> { static constexpr int E = 0, SP = 1; };
-template <> struct GridPad<2, 2> { static constexpr int E = 0, SP = 1; };
-template <> struct GridPad<2, 5> { static constexpr int E = 0, SP = 3; };
-template <> struct GridPad<2, 8> { static constexpr int E = 0, SP = 1; };
-template <> struct GridPad<2, 9> { static constexpr int E = 0, SP = 9; };
-template <> struct GridPad<3, 4> { static constexpr int E = 0, SP = 6; };
-template <> struct GridPad<3, 5> { static constexpr int E = 0, SP = 5; };
+template <> struct GridPad<1, 8> { static constexpr int E = 0, SP = 1; };
+template <> struct GridPad<2, 3> { static constexpr int E = 0, SP = 3; };
+template <> struct GridPad<2, 6> { static constexpr int E = 0, SP = 5; };
+template <> struct GridPad<2, 7> { static constexpr int E = 0, SP = 2; };
+template <> struct GridPad<2, 9> { static constexpr int E = 0, SP = 17; };
+template <> struct GridPad<3, 2> { static constexpr int E = 1, SP = 0; };
+template <> struct GridPad<3, 3> { static constexpr int E = 1, SP = 19; };
+template <> struct GridPad<3, 4> { static constexpr int E = 1, SP = 0; };
+template <> struct GridPad<3, 5> { static constexpr int E = 0, SP = 9; };
 template <> struct GridPad<3, 9> { static constexpr int E = 2, SP = 0; };
 
 template <int D, int P, int T>
@@ -260,13 +268,12 @@ struct RowLanes {
   static constexpr int SUBN = TileShape<D, P, T>::SUBN;
   static constexpr int NR = (PR + 63) / 64;                        // rest positions per lane
   static constexpr int LPS = PR < 64 ? PR : 64;                    // lanes per coil slot
-  static constexpr int BYLANES = 64 / LPS;
-  static constexpr int BYLDS = (32 * 1024) / (SUBN * 8);
-  static constexpr int NS = (BYLANES < BYLDS ? BYLANES : BYLDS) < 1 ? 1 : (BYLANES < BYLDS ? BYLANES : BYLDS);
-  // padded LDS layout: row pitch PJ (3D), pitch PI of dim 0, slot size and stride
+  static constexpr int NS = (64 / LPS) < 8 ? (64 / LPS) : 8;       // coil slots per wave
+  static constexpr int PP = ipow(SUB, D - 1);                      // dense points per plane
+  // padded LDS layout: row pitch PJ (3D), plane pitch PL, ring of P planes per slot
   static constexpr int PJ = D == 3 ? SUB + GridPad<D, P>::E : 1;
-  static constexpr int PI = D == 3 ? SUB * PJ : (D == 2 ? SUB : 1);
-  static constexpr int SLOT = PI * SUB;
+  static constexpr int PL = D == 3 ? SUB * PJ : (D == 2 ? SUB : 1);
+  static constexpr int SLOT = P * PL;
   static constexpr int SSTR = NS > 1 ? SLOT + GridPad<D, P>::SP : SLOT;
   static constexpr int LDS_FLOAT2 = (NS - 1) * SSTR + SLOT;
   // LDS stage of CH samples: rec[.][RECW] = the P dim-0 weights, the weights of dims
@@ -278,13 +285,10 @@ struct RowLanes {
   static constexpr int RECW = (P + 3) / 4 * 4;
   static constexpr int WS = CH + LA + 1;
   static constexpr int NWL = (D - 1) * P > 0 ? (D - 1) * P : 1;
-  static __device__ __forceinline__ int lds_index(int e) {         // dense subgrid index -> padded
-    if constexpr (D == 3 && GridPad<D, P>::E != 0) {
-      const int i = e / (SUB * SUB), r = e - i * SUB * SUB, j = r / SUB;
-      return i * PI + j * PJ + (r - j * SUB);
-    } else {
-      return e;
-    }
+  // dense in-plane index -> padded in-plane offset
+  static __device__ __forceinline__ int plane_off(int r) {
+    if constexpr (D == 3) return (r / SUB) * PJ + r % SUB;
+    else return r;
   }
 };
 
@@ -339,27 +343,75 @@ __device__ __forceinline__ void read_view(SampleView<D, P, RowLanes<D, P, T>::NR
   }
 }
 
-// Old subgrid values at the patch of the run starting at stage sample t.
+// Run word of a stage sample (published by the lane that gathered it): bits 0..15 the
+// padded in-plane offset of its corner, bits 16..23 its plane, bit 30 = last of its run.
+__device__ __forceinline__ int word_rest(int w) { return w & 0xffff; }
+__device__ __forceinline__ int word_plane(int w) { return (w >> 16) & 0xff; }
+
+// Ring slot offsets of the P planes of a patch whose corner plane is bi.
 template <int D, int P, int T>
-__device__ __forceinline__ void read_run(int t, int pub, const bool (&rok)[RowLanes<D, P, T>::NR],
-                                         const int (&roff)[RowLanes<D, P, T>::NR], const float2* sgrid,
-                                         float2 (&pre)[RowLanes<D, P, T>::NR][P]) {
+__device__ __forceinline__ void patch_planes(int bi, int (&off)[P]) {
   using RL = RowLanes<D, P, T>;
-  const int key = __builtin_amdgcn_readlane(pub, t) & 0xffff;
+  int s = bi % P;
 #pragma unroll
-  for (int q = 0; q < RL::NR; ++q) {
-    const float2* pt = sgrid + key + (rok[q] ? roff[q] : 0);
-#pragma unroll
-    for (int i = 0; i < P; ++i) pre[q][i] = pt[i * RL::PI];
+  for (int i = 0; i < P; ++i) {
+    off[i] = s * RL::PL;
+    s = s + 1 == P ? 0 : s + 1;
   }
 }
 
-// Accumulate stage sample t; at the end of its run (bit 30 of its published word) write
-// old + sum back, clear the sums and start reading the next run's old values.
+// Old subgrid values at the patch of the run whose word is w.
+template <int D, int P, int T>
+__device__ __forceinline__ void read_run(int w, const bool (&rok)[RowLanes<D, P, T>::NR],
+                                         const int (&roff)[RowLanes<D, P, T>::NR], const float2* sgrid,
+                                         float2 (&pre)[RowLanes<D, P, T>::NR][P]) {
+  using RL = RowLanes<D, P, T>;
+  int off[P];
+  patch_planes<D, P, T>(word_plane(w), off);
+#pragma unroll
+  for (int q = 0; q < RL::NR; ++q) {
+    const float2* pt = sgrid + word_rest(w) + (rok[q] ? roff[q] : 0);
+#pragma unroll
+    for (int i = 0; i < P; ++i) pre[q][i] = pt[off[i]];
+  }
+}
+
+// Planes lo .. hi-1 are final: store them to the scratch slots (dense) and clear their
+// ring slots, which then hold planes lo+P .. hi+P-1. Planes never held by the ring
+// (p >= lo0 + P, with lo0 the first plane not yet stored) are stored as zeros.
+template <int D, int P, int T>
+__device__ __forceinline__ void retire_planes(int lo, int hi, int nc, float2* lds2, float2* __restrict__ dst) {
+  using RL = RowLanes<D, P, T>;
+  constexpr int NE = RL::NS * RL::PP;
+  const int lane = threadIdx.x;
+  for (int p = lo; p < hi; ++p) {
+    const bool held = p < lo + P;
+    const int slot = (p % P) * RL::PL;
+#pragma unroll
+    for (int e0 = 0; e0 < NE; e0 += 64) {
+      const int e = e0 + lane;
+      const int c = e / RL::PP, r = e - c * RL::PP;
+      if (e < NE && c < nc) {
+        float2* src = lds2 + c * RL::SSTR + slot + RL::plane_off(r);
+        float2 v = make_float2(0.f, 0.f);
+        if (held) {
+          v = *src;
+          *src = make_float2(0.f, 0.f);
+        }
+        dst[(long long)c * RL::SUBN + (long long)p * RL::PP + r] = v;
+      }
+    }
+  }
+}
+
+// Accumulate stage sample t; at the end of its run (bit 30 of its word) write old + sum
+// back and clear the sums; then, before the next run's old values are read, store the
+// planes the next run has moved past.
 template <int D, int P, int T>
 __device__ __forceinline__ void accumulate_sample(const SampleView<D, P, RowLanes<D, P, T>::NR>& sv, int t, int n,
                                                   int pub, const bool (&rok)[RowLanes<D, P, T>::NR],
                                                   const int (&roff)[RowLanes<D, P, T>::NR], float2* sgrid,
+                                                  float2* lds2, float2* __restrict__ dst, int nc, int& lo,
                                                   float2 (&acc)[RowLanes<D, P, T>::NR][P],
                                                   float2 (&pre)[RowLanes<D, P, T>::NR][P]) {
   using RL = RowLanes<D, P, T>;
@@ -377,18 +429,28 @@ __device__ __forceinline__ void accumulate_sample(const SampleView<D, P, RowLane
     }
   }
   if (raw >> 30) {  // last sample of its run (uniform)
-    const int key = raw & 0xffff;
+    int off[P];
+    patch_planes<D, P, T>(word_plane(raw), off);
 #pragma unroll
     for (int q = 0; q < RL::NR; ++q) {
       if (rok[q]) {
-        float2* pt = sgrid + key + roff[q];
+        float2* pt = sgrid + word_rest(raw) + roff[q];
 #pragma unroll
-        for (int i = 0; i < P; ++i) pt[i * RL::PI] = make_float2(pre[q][i].x + acc[q][i].x, pre[q][i].y + acc[q][i].y);
+        for (int i = 0; i < P; ++i)
+          pt[off[i]] = make_float2(pre[q][i].x + acc[q][i].x, pre[q][i].y + acc[q][i].y);
       }
 #pragma unroll
       for (int i = 0; i < P; ++i) acc[q][i] = make_float2(0.f, 0.f);
     }
-    if (t + 1 < n) read_run<D, P, T>(t + 1, pub, rok, roff, sgrid, pre);
+    if (t + 1 < n) {
+      const int nw = __builtin_amdgcn_readlane(pub, t + 1);
+      const int bn = word_plane(nw);
+      if (bn > lo) {
+        retire_planes<D, P, T>(lo, bn, nc, lds2, dst);
+        lo = bn;
+      }
+      read_run<D, P, T>(nw, rok, roff, sgrid, pre);
+    }
   }
 }
 
@@ -401,8 +463,8 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
                                                           const Geom* __restrict__ geoms,
                                                           const Item* __restrict__ items, int nitems,
                                                           int nchunks, float2* __restrict__ scratch) {
-  using TS = TileShape<D, P, T>;
   using RL = RowLanes<D, P, T>;
+  constexpr int SUB = RL::SUB;
   extern __shared__ float2 lds2[];
   __shared__ float rec[RL::WS * RL::RECW];
   __shared__ float wl[RL::NWL * RL::WS];
@@ -420,8 +482,9 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
   const int cnt = it.w;
   const long long gs0 = g.samp_off + it.z;
   constexpr int CH = RL::CH;
+  float2* dst = scratch + g.scratch_off + ((long long)(iidx - g.item_base) * g.nbatch + c0) * RL::SUBN;
 
-  // first chunk's gathers go out before the subgrid is cleared
+  // first chunk's gathers go out before the ring is cleared
   StageRegs<D, P, RL::NS> sr;
   {
     const bool ok = lane < CH && lane < cnt;
@@ -430,7 +493,7 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
   }
   int onext = (lane < CH && CH + lane < cnt) ? perm[gs0 + CH + lane] : 0;
 
-  for (int e = lane; e < (nc - 1) * RL::SSTR + RL::SLOT; e += 64) lds2[e] = make_float2(0.f, 0.f);
+  for (int e = lane; e < RL::LDS_FLOAT2; e += 64) lds2[e] = make_float2(0.f, 0.f);
 
   // this lane's coil slot and rest positions
   const int cs = lane / RL::LPS;
@@ -463,12 +526,21 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
 #pragma unroll
     for (int i = 0; i < P; ++i) acc[q][i] = make_float2(0.f, 0.f);
 
-  int pub = 0;  // lane l: LDS offset of stage sample l's patch, bit 30 = last of its run
+  int lo = 0;   // lowest plane not yet stored to scratch (the ring holds lo .. lo+P-1)
+  int pub = 0;  // lane l: run word of stage sample l
   for (int s0 = 0; s0 < cnt; s0 += CH) {
     // publish the gathered chunk; a run ends where the next sample's corner differs,
     // and at the end of the stage
     {
-      const int key = RL::lds_index(sr.base);
+      int plane, rest;
+      if constexpr (D == 1) {
+        plane = sr.base;
+        rest = 0;
+      } else {
+        plane = sr.base / RL::PP;
+        rest = RL::plane_off(sr.base - plane * RL::PP);
+      }
+      const int key = rest | (plane << 16);
       const int nkey = __shfl_down(key, 1);
       const bool last = lane + 1 >= CH || s0 + lane + 1 >= cnt || nkey != key;
       pub = key | (last ? (1 << 30) : 0);
@@ -490,7 +562,15 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
       onext = (lane < CH && sl + CH < cnt) ? perm[gs0 + sl + CH] : 0;
     }
     const int n = min(CH, cnt - s0);
-    read_run<D, P, T>(0, pub, rok, roff, sgrid, pre);
+    {
+      const int w0 = __builtin_amdgcn_readlane(pub, 0);
+      const int b0 = word_plane(w0);
+      if (b0 > lo) {
+        retire_planes<D, P, T>(lo, b0, nc, lds2, dst);
+        lo = b0;
+      }
+      read_run<D, P, T>(w0, rok, roff, sgrid, pre);
+    }
     // ring of LA register sets, loaded LA-2..LA-1 samples ahead; the stage arrays are
     // padded so loads past the last sample stay in bounds (their values are unused)
     SampleView<D, P, RL::NR> ring[RL::LA];
@@ -499,19 +579,15 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
     for (int t = 0; t < n; t += RL::LA) {
 #pragma unroll
       for (int u = 0; u < RL::LA; ++u) {
-        if (t + u < n) accumulate_sample<D, P, T>(ring[u], t + u, n, pub, rok, roff, sgrid, acc, pre);
+        if (t + u < n)
+          accumulate_sample<D, P, T>(ring[u], t + u, n, pub, rok, roff, sgrid, lds2, dst, nc, lo, acc, pre);
         read_view<D, P, T>(ring[(u + RL::LA - 1) % RL::LA], t + u + RL::LA - 1, rec, wl, vmine, ridx);
       }
     }
     __syncthreads();
   }
-  float2* dst = scratch + g.scratch_off + ((long long)(iidx - g.item_base) * g.nbatch + c0) * TS::SUBN;
-  for (int c = 0; c < nc; ++c) {
-    const float2* src = lds2 + c * RL::SSTR;
-    float2* d = dst + c * TS::SUBN;
-#pragma unroll 8
-    for (int e = lane; e < TS::SUBN; e += 64) d[e] = src[RL::lds_index(e)];
-  }
+  // the remaining planes (held or never reached) are final
+  retire_planes<D, P, T>(lo, SUB, nc, lds2, dst);
 }
 
 // Gridding phase 2. One block per (problem, output tile, group of RB batch entries):
