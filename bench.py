@@ -1,5 +1,10 @@
 """Benchmark: 3D radial kooshball SENSE NUFFT forward+adjoint on one MI355X.
 
+Other workloads (``--workload``): ``batched`` -- 64 uneven 2D CG-SENSE problems in one
+captured graph against a loop of plain single-problem calls; ``grad`` -- loss and
+gradients (image and trajectory) through a 2D SENSE NUFFT plus an unrolled CG-SENSE
+against the plain per-coil differentiable path. Each prints its own JSON line.
+
 Workload (the headline configuration): 128^3 complex64 image, 32 smooth coil
 maps, 20,000 Fibonacci-sphere spokes x 256 readout points (5.12 M samples per
 coil), Kaiser-Bessel width 4, oversampling 1.25 (grid 160^3). One step = one
@@ -115,12 +120,201 @@ def cpu_baseline(frac_spokes=1 / 4):
     }
 
 
+def _timed(fn, reps):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps, out
+
+
+def run_batched(args):
+    """64 uneven 2D problems: N in {128,192,256}, C in {8,16,32}, radial or spiral, 20k..130k
+    samples per coil; 10 CG-SENSE iterations each (lamda 1e-3, max_eigen 1)."""
+    import numpy as np
+
+    from mr_recon_amd import synth
+    from mr_recon_amd.batched import BatchedSense, Problem
+    from mr_recon_amd.fourier import hip_nufft
+    from mr_recon_amd.linops import sense_linop
+    from mr_recon_amd.recons import CG_SENSE_recon
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(2024)
+    probs, ksp, spec = [], [], []
+    for i in range(64):
+        N = int(rng.choice([128, 192, 256]))
+        C = int(rng.choice([8, 16, 32]))
+        M = int(rng.integers(20000, 130001))
+        if rng.random() < 0.5:
+            nread = 2 * N
+            trj = synth.radial_2d(-(-M // nread), nread, N)
+        else:
+            trj = synth.spiral_2d(16, -(-M // 16), N)
+        trj = trj.reshape(-1, 2)
+        mps = synth.coil_maps(C, (N, N), seed=i)
+        probs.append(Problem(trj.to(dev), mps.to(dev)))
+        spec.append((N, C, trj.shape[0]))
+    kw = dict(num_iters=10, lamda_l2=1e-3, max_eigen=1.0)
+    t0 = time.time()
+    B = BatchedSense(probs, oversamp=1.25, width=4, samples_per_item=args.samples_per_item)
+    torch.cuda.synchronize()
+    log(f"batched operators built in {time.time() - t0:.1f}s ({len(B.groups)} groups)")
+    # the plain path gets the same (autotuned) work-item size per problem, so both paths
+    # sum in the same order and their reconstructions can be compared bit for bit
+    S_of = {i: plan.S for g in B.groups for i, plan in zip(g.idx, g.plans)}
+    t0 = time.time()
+    plain_ops = [sense_linop(p.trj, p.mps, nufft=hip_nufft((n, n), oversamp=1.25, width=4, samples_per_item=S_of[i]))
+                 for i, (p, (n, _, _)) in enumerate(zip(probs, spec))]
+    torch.cuda.synchronize()
+    log(f"plain operators built in {time.time() - t0:.1f}s")
+    # k-space of a phantom with random phase plus complex noise at ~30 dB SNR
+    for i, (A, (n, c, m)) in enumerate(zip(plain_ops, spec)):
+        y = A.forward(synth.phantom_2d(n, seed=i).to(dev))
+        sigma = float(torch.linalg.vector_norm(y)) / (y.numel() ** 0.5) * 10 ** (-30 / 20)
+        ksp.append(y + synth.complex_noise(y.shape, sigma, seed=2000 + i).to(dev))
+
+    def plain():
+        return [CG_SENSE_recon(A, y, max_iter=10, lamda_l2=1e-3, max_eigen=1.0, tolerance=0.0, verbose=False)
+                for A, y in zip(plain_ops, ksp)]
+
+    t0 = time.time()
+    B.capture(**kw)
+    torch.cuda.synchronize()
+    log(f"graph captured in {time.time() - t0:.1f}s")
+    for _ in range(args.warmup):
+        plain()
+        B.replay(ksp)
+    t_plain, ref = _timed(plain, max(1, args.steps // 4))
+    t_batch, got = _timed(lambda: [x.clone() for x in B.replay(ksp)], args.steps)
+    err = max(float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)) for a, b in zip(got, ref))
+    rec = {
+        "metric": "CG-SENSE reconstructions/s (64 uneven 2D problems, 10 CG iterations each)",
+        "value": 64 / t_batch, "unit": "recons/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64", "data": "synthetic (seeded radial/spiral trajectories, Gaussian-bump coil maps, phantom k-space + 30 dB noise)",
+        "config": {"workload": "64 uneven 2D CG-SENSE problems, one captured graph", "N": [128, 192, 256],
+                   "coils": [8, 16, 32], "samples_per_coil": [20000, 130000], "cg_iters": 10, "width": 4,
+                   "oversamp": 1.25, "samples_per_item": str(args.samples_per_item)},
+        "plain_loop": {"value": 64 / t_plain, "unit": "recons/s", "ms": t_plain * 1e3,
+                       "path": "per problem: sense_linop + CG_SENSE_recon (fused single-problem HIP operators)"},
+        "speedup_vs_plain": t_plain / t_batch,
+        "max_rel_l2_vs_plain": err,
+        "total_samples": int(sum(c * m for _, c, m in spec)),
+    }
+    print(json.dumps(rec), flush=True)
+
+
+def run_grad(args):
+    """loss = ||A(k) x - y||^2 + ||CG_10(A(k), y) - x0||^2, 256^2 image, 16 coils, 64 spiral
+    interleaves x 2048; gradients w.r.t. x and the trajectory k."""
+    from mr_recon_amd import diff, synth
+    from mr_recon_amd.fourier import hip_nufft
+
+    dev = torch.device("cuda:0")
+    N, C = 256, 16
+    k0 = synth.spiral_2d(64, 2048, N).to(dev)
+    mps = synth.coil_maps(C, (N, N), seed=5).to(dev)
+    x0 = synth.phantom_2d(N, seed=5).to(dev)
+    nf = hip_nufft((N, N), oversamp=1.25, width=6)
+    with torch.no_grad():
+        y = diff.sense_forward(x0, k0, mps, nf)
+        y = y + 0.01 * synth.random_image(y.shape, seed=9).to(dev) * y.abs().mean()
+    xin = (x0 * 0.9).detach()
+    kin = (k0 + 0.05).detach()
+    lam, eig, iters = 1e-3, 1.0, 10
+
+    def fused():
+        x = xin.clone().requires_grad_(True)
+        k = kin.clone().requires_grad_(True)
+        dc = torch.sum(torch.abs(diff.sense_forward(x, k, mps, nf) - y) ** 2)
+        rec = diff.unrolled_cg_sense(y, k, mps, nf, num_iters=iters, lamda_l2=lam, max_eigen=eig, checkpoint=True)
+        loss = dc + torch.sum(torch.abs(rec - x0) ** 2)
+        loss.backward()
+        return loss.detach(), x.grad, k.grad
+
+    def plain():
+        """Per-coil differentiable NUFFTs composed in Python, unrolled CG without checkpointing."""
+        x = xin.clone().requires_grad_(True)
+        k = kin.clone().requires_grad_(True)
+
+        def A(v):
+            return [diff.nufft_forward(mps[c] * v, k, nf) for c in range(C)]
+
+        def AH(ys):
+            return sum(mps[c].conj() * diff.nufft_adjoint(ys[c], k, nf) for c in range(C))
+
+        dc = sum(torch.sum(torch.abs(a - y[c]) ** 2) for c, a in enumerate(A(x)))
+        inv = 1.0 / eig
+        ahb = AH([y[c] for c in range(C)]) * eig ** -0.5
+        xx = ahb
+        r = ahb - (AH(A(xx)) * inv + lam * xx)
+        p = r
+        rz = torch.real(torch.sum(r.conj() * r))
+        for _ in range(iters):
+            ap = AH(A(p)) * inv + lam * p
+            a = rz / torch.real(torch.sum(p.conj() * ap))
+            xx = xx + a * p
+            r = r - a * ap
+            rzn = torch.real(torch.sum(r.conj() * r))
+            p = r + (rzn / rz) * p
+            rz = rzn
+        loss = dc + torch.sum(torch.abs(xx - x0) ** 2)
+        loss.backward()
+        return loss.detach(), x.grad, k.grad
+
+    for _ in range(args.warmup):
+        fused()
+        plain()
+    t_f, (lf, gxf, gkf) = _timed(fused, args.steps)
+    t_p, (lp, gxp, gkp) = _timed(plain, max(1, args.steps // 2))
+
+    def rel(a, b):
+        return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+    # the data-consistency term alone: no CG, so no growth of rounding differences
+    def dc_only(fused_path):
+        x = xin.clone().requires_grad_(True)
+        k = kin.clone().requires_grad_(True)
+        if fused_path:
+            loss = torch.sum(torch.abs(diff.sense_forward(x, k, mps, nf) - y) ** 2)
+        else:
+            loss = sum(torch.sum(torch.abs(diff.nufft_forward(mps[c] * x, k, nf) - y[c]) ** 2) for c in range(C))
+        loss.backward()
+        return loss.detach(), x.grad, k.grad
+
+    ldf, gxdf, gkdf = dc_only(True)
+    ldp, gxdp, gkdp = dc_only(False)
+
+    rec = {
+        "metric": "loss+gradient evaluations/s (2D 256^2 x 16 coils SENSE NUFFT + 10-iteration unrolled CG)",
+        "value": 1.0 / t_f, "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_f * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64", "data": "synthetic (seeded spiral, phantom with random phase, Gaussian-bump coil maps)",
+        "config": {"workload": "dL/dx and dL/dk of ||A(k)x-y||^2 + ||CG10(A(k),y)-x0||^2", "image": [N, N],
+                   "coils": C, "interleaves": 64, "samples_per_interleave": 2048, "width": 6, "oversamp": 1.25,
+                   "cg_iters": iters, "checkpoint": True},
+        "plain_path": {"value": 1.0 / t_p, "unit": "evals/s", "ms": t_p * 1e3,
+                       "path": "per-coil differentiable NUFFTs, Python-composed SENSE and CG, no checkpointing"},
+        "speedup_vs_plain": t_p / t_f,
+        "agreement": {"loss": abs(float(lf - lp)) / abs(float(lp)), "grad_x": rel(gxf, gxp),
+                      "grad_k": rel(gkf, gkp),
+                      "data_term_only": {"loss": abs(float(ldf - ldp)) / abs(float(ldp)), "grad_x": rel(gxdf, gxdp),
+                                         "grad_k": rel(gkdf, gkdp)}},
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="nufft3d", choices=["nufft3d", "batched", "grad"])
+    ap.add_argument("--samples-per-item", default="auto",
+                    help="batched workload: work-item size (an int, or 'auto' for the autotuner)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, run one elementwise kernel that reads 1 GiB and writes 1 GiB "
                          "(calibrates the HBM byte counters)")
@@ -129,6 +323,12 @@ def main():
         print("bench.py measures one GPU only: this build runs the NUFFT on a single MI355X "
               "(multi-GPU sharding is a design note, not implemented). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        if args.samples_per_item != "auto":
+            args.samples_per_item = int(args.samples_per_item)
+        return run_batched(args)
+    if args.workload == "grad":
+        return run_grad(args)
 
     from mr_recon_amd.fourier import hip_nufft
     from mr_recon_amd.linops import sense_linop

@@ -54,7 +54,9 @@ class _Group:
         self.d = 2
         dev = problems[idx[0]].trj.device
         self.device = dev
-        self.nufft = hip_nufft(self.im, oversamp=oversamp, width=width, samples_per_item=samples_per_item)
+        cmax = max(int(problems[i].mps.shape[0]) for i in idx)
+        self.nufft = hip_nufft(self.im, oversamp=oversamp, width=width, samples_per_item=samples_per_item,
+                               tune_batch=cmax)
         self.G = self.nufft.os_shape()
         self.apod = self.nufft.apod_table(dev)
         self.s_fwd = 1.0 / math.sqrt(N * N)
