@@ -111,6 +111,14 @@ int mra_cmul_bcast(void* data, const void* k, long long nbatch, long long n, voi
 int mra_fft_plan(int ndim, const int* n, int batch, void** plan);
 int mra_fft_exec(void* plan, void* data, int inverse, void* stream);
 int mra_fft_destroy(void* plan);
+/* The same transforms with one LDS-staged Stockham pass per axis (lengths whose radix plan
+ * is compiled in: 64, 80, 96, 120, 128, 160, 192, 240, 256, 320, 384, 480, 512, 640; every
+ * axis but the last needs >= 64 points after it). tw[d] is the device table of
+ * exp(-2 pi i m / n[d]), m < n[d], filled on the host by mra_fft_lds_twiddles. */
+int mra_fft_lds_supported(int ndim, const int* n);
+int mra_fft_lds_twiddles(int L, float* tw_host);
+int mra_fft_lds(int ndim, const int* n, long long batch, void* data, const void* const* tw, int inverse,
+                void* stream);
 
 /* ---- conjugate gradient -----------------------------------------------------------------
  * Vector work of conjugate_gradient (algs.py:779-872) with FP64 inner products

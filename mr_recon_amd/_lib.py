@@ -56,6 +56,9 @@ SIGNATURES = {
     "mra_fft_plan": (_i, [_i, _ip, _i, C.POINTER(_vp)]),
     "mra_fft_exec": (_i, [_vp, _vp, _i, _vp]),
     "mra_fft_destroy": (_i, [_vp]),
+    "mra_fft_lds_supported": (_i, [_i, _ip]),
+    "mra_fft_lds_twiddles": (_i, [_i, _fp]),
+    "mra_fft_lds": (_i, [_i, _ip, _ll, _vp, C.POINTER(_vp), _i, _vp]),
     "mra_cg_partials": (_i, []),
     "mra_cg_seg_blocks": (_i, []),
     "mra_cdot_re": (_i, [_vp, _vp, _ll, _f, _vp, _vp, _i, _vp]),

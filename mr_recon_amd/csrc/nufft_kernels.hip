@@ -49,7 +49,7 @@ template <> struct TileCfg<1> { static constexpr int T = 256; };
 template <> struct TileCfg<2> { static constexpr int T = 16; };
 template <> struct TileCfg<3> { static constexpr int T = 8; };
 
-constexpr int kLdsBudget = 64 * 1024;  // bytes of LDS subgrids per block
+constexpr int kLdsBudget = 32 * 1024;  // bytes of LDS subgrids per interp block (more, smaller blocks hide latency better)
 
 // Offset in the subgrid of patch point q of the trailing D-1 dims.
 template <int D, int P, int T>
@@ -247,6 +247,13 @@ __global__ __launch_bounds__(kThreads) void interp_kernel(const float2* __restri
 // Scratch slot of (item, batch entry): scratch + scratch_off + ((item - item_base) * nbatch + b) * SUBN,
 // dense row-major subgrid.
 
+#ifndef MRA_GRID_STAGE
+#define MRA_GRID_STAGE 64
+#endif
+#ifndef MRA_GRID_RING
+#define MRA_GRID_RING 6
+#endif
+
 // LDS padding (row pitch of the last dim += E in 3D, coil-slot stride += SP), from a
 // bank model of the run read-modify-write (64 banks, 8-byte lanes) per (D, P).
 template <int D, int P> struct GridPad { static constexpr int E = 0, SP = 0; };
@@ -280,8 +287,8 @@ struct RowLanes {
   // 1..D-1 as [(D-1)*P][WS] and the k-space values as float2 [NS][WS]. WS is odd so the
   // P weights one lane may pick for a sample fall in distinct banks, and covers the
   // LA-1 loads the ring issues past the end of a stage.
-  static constexpr int CH = 32;                                    // samples per stage
-  static constexpr int LA = 4;                                     // register ring depth
+  static constexpr int CH = MRA_GRID_STAGE;                        // samples per stage
+  static constexpr int LA = MRA_GRID_RING;                         // register ring depth
   static constexpr int RECW = (P + 3) / 4 * 4;
   static constexpr int WS = CH + LA + 1;
   static constexpr int NWL = (D - 1) * P > 0 ? (D - 1) * P : 1;
