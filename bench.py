@@ -3,7 +3,9 @@
 Other workloads (``--workload``): ``batched`` -- 64 uneven 2D CG-SENSE problems in one
 captured graph against a loop of plain single-problem calls; ``grad`` -- loss and
 gradients (image and trajectory) through a 2D SENSE NUFFT plus an unrolled CG-SENSE
-against the plain per-coil differentiable path. Each prints its own JSON line.
+against the plain per-coil differentiable path; ``cg2d`` -- 2D spiral CG-SENSE, 30
+iterations, Toeplitz normal operator; ``cg3d`` -- the 3D kooshball above, 15 CG-SENSE
+iterations (Toeplitz and NUFFT normal operators). Each prints its own JSON line.
 
 Workload (the headline configuration): 128^3 complex64 image, 32 smooth coil
 maps, 20,000 Fibonacci-sphere spokes x 256 readout points (5.12 M samples per
@@ -306,13 +308,73 @@ def run_grad(args):
     print(json.dumps(rec), flush=True)
 
 
+def run_cg(args, three_d: bool):
+    """CG-SENSE reconstructions: 2D 256^2 x 16 coils, 32 spiral interleaves x 4096, 30 CG
+    iterations (Toeplitz normal operator), or 3D 128^3 x 32 coils kooshball, 15 iterations
+    (Toeplitz and NUFFT normal operators); lamda 1e-3, max_eigen 1; data = A(phantom) + noise."""
+    from mr_recon_amd import synth
+    from mr_recon_amd.fourier import hip_nufft
+    from mr_recon_amd.linops import sense_linop
+    from mr_recon_amd.recons import CG_SENSE_recon
+
+    dev = torch.device("cuda:0")
+    if three_d:
+        n, c, iters, w = N, C, 15, WIDTH
+        trj = synth.kooshball_3d(NSPOKES, NREAD, n).to(dev)
+        x_true = synth.random_image((n,) * 3, seed=1).to(dev)
+        name = f"3D {n}^3 x {c} coils, {NSPOKES} kooshball spokes x {NREAD}"
+    else:
+        n, c, iters, w = 256, 16, 30, 4
+        trj = synth.spiral_2d(32, 4096, n).to(dev)
+        x_true = synth.phantom_2d(n, seed=3).to(dev)
+        name = f"2D {n}^2 x {c} coils, 32 spiral interleaves x 4096"
+    mps = synth.coil_maps(c, x_true.shape, seed=2).to(dev)
+    res, xs = {}, {}
+    for toep in ((True, False) if three_d else (True,)):
+        t0 = time.time()
+        A = sense_linop(trj, mps, nufft=hip_nufft(x_true.shape, oversamp=OS, width=w), use_toeplitz=toep)
+        torch.cuda.synchronize()
+        setup = time.time() - t0
+        y = A.forward(x_true)
+        sigma = float(torch.linalg.vector_norm(y)) / (y.numel() ** 0.5) * 10 ** (-30 / 20)
+        y = y + synth.complex_noise(y.shape, sigma, seed=4).to(dev)
+
+        def recon():
+            return CG_SENSE_recon(A, y, max_iter=iters, lamda_l2=1e-3, max_eigen=1.0, tolerance=0.0, verbose=False)
+
+        for _ in range(args.warmup):
+            recon()
+        t, x = _timed(recon, args.steps)
+        key = "toeplitz" if toep else "nufft_normal"
+        xs[key] = x
+        res[key] = {"ms_per_recon": t * 1e3, "ms_per_iter": t * 1e3 / iters, "setup_s": round(setup, 3)}
+        if not three_d:  # the 2D phantom is recoverable; the 3D random image is not in 15 iterations
+            res[key]["rel_err_vs_truth"] = float(torch.linalg.vector_norm(x - x_true) / torch.linalg.vector_norm(x_true))
+        log(f"{key}: {t * 1e3:.2f} ms per recon, setup {setup:.2f}s")
+        del A
+    if len(xs) == 2:
+        a, b = xs["toeplitz"], xs["nufft_normal"]
+        res["toeplitz_vs_nufft_normal_rel_l2"] = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+    best = min(v["ms_per_recon"] for v in res.values() if isinstance(v, dict))
+    rec = {
+        "metric": f"CG-SENSE reconstructions/s ({name}, {iters} CG iterations)",
+        "value": 1e3 / best, "unit": "recons/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": best, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c64",
+        "data": "synthetic (seeded trajectory, Gaussian-bump coil maps, image + 30 dB noise)",
+        "config": {"workload": "CG-SENSE", "image": list(x_true.shape), "coils": c, "cg_iters": iters, "width": w,
+                   "oversamp": OS, "lamda_l2": 1e-3, "max_eigen": 1.0},
+        "variants": res,
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="nufft3d", choices=["nufft3d", "batched", "grad"])
+    ap.add_argument("--workload", default="nufft3d", choices=["nufft3d", "batched", "grad", "cg2d", "cg3d"])
     ap.add_argument("--samples-per-item", default="auto",
                     help="batched workload: work-item size (an int, or 'auto' for the autotuner)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -329,6 +391,8 @@ def main():
         return run_batched(args)
     if args.workload == "grad":
         return run_grad(args)
+    if args.workload in ("cg2d", "cg3d"):
+        return run_cg(args, args.workload == "cg3d")
 
     from mr_recon_amd.fourier import hip_nufft
     from mr_recon_amd.linops import sense_linop

@@ -253,6 +253,9 @@ __global__ __launch_bounds__(kThreads) void interp_kernel(const float2* __restri
 #ifndef MRA_GRID_RING
 #define MRA_GRID_RING 6
 #endif
+#ifndef MRA_GRID_CPL
+#define MRA_GRID_CPL 1
+#endif
 
 // LDS padding (row pitch of the last dim += E in 3D, coil-slot stride += SP), from a
 // bank model of the run read-modify-write (64 banks, 8-byte lanes) per (D, P).
@@ -275,16 +278,19 @@ struct RowLanes {
   static constexpr int SUBN = TileShape<D, P, T>::SUBN;
   static constexpr int NR = (PR + 63) / 64;                        // rest positions per lane
   static constexpr int LPS = PR < 64 ? PR : 64;                    // lanes per coil slot
-  static constexpr int NS = (64 / LPS) < 8 ? (64 / LPS) : 8;       // coil slots per wave
+  static constexpr int NS = (64 / LPS) < 8 ? (64 / LPS) : 8;       // lane groups per wave
+  static constexpr int CPL = MRA_GRID_CPL;                         // coils per lane
+  static constexpr int NC = NS * CPL;                              // coils per wave (lane group cs
+                                                                   // owns coils cs + u*NS, u < CPL)
   static constexpr int PP = ipow(SUB, D - 1);                      // dense points per plane
   // padded LDS layout: row pitch PJ (3D), plane pitch PL, ring of P planes per slot
   static constexpr int PJ = D == 3 ? SUB + GridPad<D, P>::E : 1;
   static constexpr int PL = D == 3 ? SUB * PJ : (D == 2 ? SUB : 1);
   static constexpr int SLOT = P * PL;
   static constexpr int SSTR = NS > 1 ? SLOT + GridPad<D, P>::SP : SLOT;
-  static constexpr int LDS_FLOAT2 = (NS - 1) * SSTR + SLOT;
+  static constexpr int LDS_FLOAT2 = (NC - 1) * SSTR + SLOT;
   // LDS stage of CH samples: rec[.][RECW] = the P dim-0 weights, the weights of dims
-  // 1..D-1 as [(D-1)*P][WS] and the k-space values as float2 [NS][WS]. WS is odd so the
+  // 1..D-1 as [(D-1)*P][WS] and the k-space values as float2 [NC][WS]. WS is odd so the
   // P weights one lane may pick for a sample fall in distinct banks, and covers the
   // LA-1 loads the ring issues past the end of a stage.
   static constexpr int CH = MRA_GRID_STAGE;                        // samples per stage
@@ -307,7 +313,7 @@ struct StageRegs {
 };
 
 template <int D, int P, int T>
-__device__ __forceinline__ void stage_load(StageRegs<D, P, RowLanes<D, P, T>::NS>& r, bool ok, long long gs, int o,
+__device__ __forceinline__ void stage_load(StageRegs<D, P, RowLanes<D, P, T>::NC>& r, bool ok, long long gs, int o,
                                            const float2* __restrict__ ksp, const float* __restrict__ dcf,
                                            const float* __restrict__ wsort, const int* __restrict__ lbase,
                                            const Geom& g, int c0, int nc) {
@@ -319,7 +325,7 @@ __device__ __forceinline__ void stage_load(StageRegs<D, P, RowLanes<D, P, T>::NS
   const float sc = (ok && dcf) ? dcf[g.samp_off + o] * g.out_scale : g.out_scale;
   const float2* kc = ksp + g.ksp_off + (long long)c0 * g.ksp_stride + (ok ? o : 0);
 #pragma unroll
-  for (int cs = 0; cs < RL::NS; ++cs) {
+  for (int cs = 0; cs < RL::NC; ++cs) {
     float2 v = (ok && cs < nc) ? kc[(long long)cs * g.ksp_stride] : make_float2(0.f, 0.f);
     r.v[cs] = make_float2(v.x * sc, v.y * sc);
   }
@@ -327,22 +333,24 @@ __device__ __forceinline__ void stage_load(StageRegs<D, P, RowLanes<D, P, T>::NS
 
 // Stage values of one sample as loaded (no arithmetic on them until they are used,
 // so later samples' loads stay in flight while this one is accumulated).
-template <int D, int P, int NR>
+template <int D, int P, int NR, int CPL>
 struct SampleView {
   float wx[P];           // dim-0 weights
   float wa[NR], wb[NR];  // weights of dims 1 and 2 at this lane's rest positions
-  float2 v;
+  float2 v[CPL];         // k-space values of this lane's coils
 };
 
 template <int D, int P, int T>
-__device__ __forceinline__ void read_view(SampleView<D, P, RowLanes<D, P, T>::NR>& sv, int t, const float* rec,
+__device__ __forceinline__ void read_view(SampleView<D, P, RowLanes<D, P, T>::NR, RowLanes<D, P, T>::CPL>& sv,
+                                          int t, const float* rec,
                                           const float* wl, const float2* vmine,
                                           const int (&ridx)[RowLanes<D, P, T>::NR][D > 1 ? D - 1 : 1]) {
   using RL = RowLanes<D, P, T>;
   const float* rt = rec + t * RL::RECW;
 #pragma unroll
   for (int i = 0; i < P; ++i) sv.wx[i] = rt[i];
-  sv.v = vmine[t];
+#pragma unroll
+  for (int u = 0; u < RL::CPL; ++u) sv.v[u] = vmine[u * RL::NS * RL::WS + t];
 #pragma unroll
   for (int q = 0; q < RL::NR; ++q) {
     if constexpr (D >= 2) sv.wa[q] = wl[ridx[q][0] * RL::WS + t];
@@ -371,16 +379,18 @@ __device__ __forceinline__ void patch_planes(int bi, int (&off)[P]) {
 template <int D, int P, int T>
 __device__ __forceinline__ void read_run(int w, const bool (&rok)[RowLanes<D, P, T>::NR],
                                          const int (&roff)[RowLanes<D, P, T>::NR], const float2* sgrid,
-                                         float2 (&pre)[RowLanes<D, P, T>::NR][P]) {
+                                         float2 (&pre)[RowLanes<D, P, T>::CPL][RowLanes<D, P, T>::NR][P]) {
   using RL = RowLanes<D, P, T>;
   int off[P];
   patch_planes<D, P, T>(word_plane(w), off);
 #pragma unroll
-  for (int q = 0; q < RL::NR; ++q) {
-    const float2* pt = sgrid + word_rest(w) + (rok[q] ? roff[q] : 0);
+  for (int u = 0; u < RL::CPL; ++u)
 #pragma unroll
-    for (int i = 0; i < P; ++i) pre[q][i] = pt[off[i]];
-  }
+    for (int q = 0; q < RL::NR; ++q) {
+      const float2* pt = sgrid + u * RL::NS * RL::SSTR + word_rest(w) + (rok[q] ? roff[q] : 0);
+#pragma unroll
+      for (int i = 0; i < P; ++i) pre[u][q][i] = pt[off[i]];
+    }
 }
 
 // Planes lo .. hi-1 are final: store them to the scratch slots (dense) and clear their
@@ -389,7 +399,7 @@ __device__ __forceinline__ void read_run(int w, const bool (&rok)[RowLanes<D, P,
 template <int D, int P, int T>
 __device__ __forceinline__ void retire_planes(int lo, int hi, int nc, float2* lds2, float2* __restrict__ dst) {
   using RL = RowLanes<D, P, T>;
-  constexpr int NE = RL::NS * RL::PP;
+  constexpr int NE = RL::NC * RL::PP;
   const int lane = threadIdx.x;
   for (int p = lo; p < hi; ++p) {
     const bool held = p < lo + P;
@@ -415,12 +425,12 @@ __device__ __forceinline__ void retire_planes(int lo, int hi, int nc, float2* ld
 // back and clear the sums; then, before the next run's old values are read, store the
 // planes the next run has moved past.
 template <int D, int P, int T>
-__device__ __forceinline__ void accumulate_sample(const SampleView<D, P, RowLanes<D, P, T>::NR>& sv, int t, int n,
-                                                  int pub, const bool (&rok)[RowLanes<D, P, T>::NR],
-                                                  const int (&roff)[RowLanes<D, P, T>::NR], float2* sgrid,
-                                                  float2* lds2, float2* __restrict__ dst, int nc, int& lo,
-                                                  float2 (&acc)[RowLanes<D, P, T>::NR][P],
-                                                  float2 (&pre)[RowLanes<D, P, T>::NR][P]) {
+__device__ __forceinline__ void accumulate_sample(
+    const SampleView<D, P, RowLanes<D, P, T>::NR, RowLanes<D, P, T>::CPL>& sv, int t, int n, int pub,
+    const bool (&rok)[RowLanes<D, P, T>::CPL][RowLanes<D, P, T>::NR], const int (&roff)[RowLanes<D, P, T>::NR],
+    float2* sgrid, float2* lds2, float2* __restrict__ dst, int nc, int& lo,
+    float2 (&acc)[RowLanes<D, P, T>::CPL][RowLanes<D, P, T>::NR][P],
+    float2 (&pre)[RowLanes<D, P, T>::CPL][RowLanes<D, P, T>::NR][P]) {
   using RL = RowLanes<D, P, T>;
   const int raw = __builtin_amdgcn_readlane(pub, t);  // scalar, no LDS round trip
 #pragma unroll
@@ -428,27 +438,32 @@ __device__ __forceinline__ void accumulate_sample(const SampleView<D, P, RowLane
     float wr = 1.0f;
     if constexpr (D == 3) wr = sv.wa[q] * sv.wb[q];
     else if constexpr (D == 2) wr = sv.wa[q];
-    const float vx = sv.v.x * wr, vy = sv.v.y * wr;
 #pragma unroll
-    for (int i = 0; i < P; ++i) {
-      acc[q][i].x += sv.wx[i] * vx;
-      acc[q][i].y += sv.wx[i] * vy;
+    for (int u = 0; u < RL::CPL; ++u) {
+      const float vx = sv.v[u].x * wr, vy = sv.v[u].y * wr;
+#pragma unroll
+      for (int i = 0; i < P; ++i) {
+        acc[u][q][i].x += sv.wx[i] * vx;
+        acc[u][q][i].y += sv.wx[i] * vy;
+      }
     }
   }
   if (raw >> 30) {  // last sample of its run (uniform)
     int off[P];
     patch_planes<D, P, T>(word_plane(raw), off);
 #pragma unroll
-    for (int q = 0; q < RL::NR; ++q) {
-      if (rok[q]) {
-        float2* pt = sgrid + word_rest(raw) + roff[q];
+    for (int u = 0; u < RL::CPL; ++u)
 #pragma unroll
-        for (int i = 0; i < P; ++i)
-          pt[off[i]] = make_float2(pre[q][i].x + acc[q][i].x, pre[q][i].y + acc[q][i].y);
+      for (int q = 0; q < RL::NR; ++q) {
+        if (rok[u][q]) {
+          float2* pt = sgrid + u * RL::NS * RL::SSTR + word_rest(raw) + roff[q];
+#pragma unroll
+          for (int i = 0; i < P; ++i)
+            pt[off[i]] = make_float2(pre[u][q][i].x + acc[u][q][i].x, pre[u][q][i].y + acc[u][q][i].y);
+        }
+#pragma unroll
+        for (int i = 0; i < P; ++i) acc[u][q][i] = make_float2(0.f, 0.f);
       }
-#pragma unroll
-      for (int i = 0; i < P; ++i) acc[q][i] = make_float2(0.f, 0.f);
-    }
     if (t + 1 < n) {
       const int nw = __builtin_amdgcn_readlane(pub, t + 1);
       const int bn = word_plane(nw);
@@ -456,7 +471,7 @@ __device__ __forceinline__ void accumulate_sample(const SampleView<D, P, RowLane
         retire_planes<D, P, T>(lo, bn, nc, lds2, dst);
         lo = bn;
       }
-      read_run<D, P, T>(nw, rok, roff, sgrid, pre);
+      read_run<D, P, T>(nw, rok[0], roff, sgrid, pre);
     }
   }
 }
@@ -475,16 +490,16 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
   extern __shared__ float2 lds2[];
   __shared__ float rec[RL::WS * RL::RECW];
   __shared__ float wl[RL::NWL * RL::WS];
-  __shared__ float2 vst[RL::NS * RL::WS];
+  __shared__ float2 vst[RL::NC * RL::WS];
   int iidx, cc;
   item_of_block(blockIdx.x, nchunks, iidx, cc);
   if (iidx >= nitems) return;
   const Item it = items[iidx];
   if (it.w <= 0 || it.y < 0) return;  // non-finite samples contribute nothing
   const Geom g = geoms[it.x];
-  const int c0 = cc * RL::NS;
+  const int c0 = cc * RL::NC;
   if (c0 >= g.nbatch) return;
-  const int nc = min(RL::NS, g.nbatch - c0);
+  const int nc = min(RL::NC, g.nbatch - c0);
   const int lane = threadIdx.x;
   const int cnt = it.w;
   const long long gs0 = g.samp_off + it.z;
@@ -492,7 +507,7 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
   float2* dst = scratch + g.scratch_off + ((long long)(iidx - g.item_base) * g.nbatch + c0) * RL::SUBN;
 
   // first chunk's gathers go out before the ring is cleared
-  StageRegs<D, P, RL::NS> sr;
+  StageRegs<D, P, RL::NC> sr;
   {
     const bool ok = lane < CH && lane < cnt;
     const int o = ok ? perm[gs0 + lane] : 0;
@@ -507,12 +522,13 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
   const int rl = lane - cs * RL::LPS;
   const bool active = cs < nc;
   int roff[RL::NR], ridx[RL::NR][D > 1 ? D - 1 : 1];
-  bool rok[RL::NR];
+  bool rok[RL::CPL][RL::NR];
 #pragma unroll
   for (int q = 0; q < RL::NR; ++q) {
     const int r = rl + q * 64;
-    rok[q] = active && r < RL::PR;
-    const int rr = rok[q] ? r : 0;
+#pragma unroll
+    for (int u = 0; u < RL::CPL; ++u) rok[u][q] = cs + u * RL::NS < nc && cs < RL::NS && r < RL::PR;
+    const int rr = rok[0][q] ? r : 0;
     if constexpr (D == 3) {
       ridx[q][0] = rr / P;
       ridx[q][1] = rr % P;
@@ -527,11 +543,13 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
   }
   float2* sgrid = lds2 + (active ? cs : 0) * RL::SSTR;
   const float2* vmine = vst + (active ? cs : 0) * RL::WS;
-  float2 acc[RL::NR][P], pre[RL::NR][P];
+  float2 acc[RL::CPL][RL::NR][P], pre[RL::CPL][RL::NR][P];
 #pragma unroll
-  for (int q = 0; q < RL::NR; ++q)
+  for (int u = 0; u < RL::CPL; ++u)
 #pragma unroll
-    for (int i = 0; i < P; ++i) acc[q][i] = make_float2(0.f, 0.f);
+    for (int q = 0; q < RL::NR; ++q)
+#pragma unroll
+      for (int i = 0; i < P; ++i) acc[u][q][i] = make_float2(0.f, 0.f);
 
   int lo = 0;   // lowest plane not yet stored to scratch (the ring holds lo .. lo+P-1)
   int pub = 0;  // lane l: run word of stage sample l
@@ -559,7 +577,7 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
 #pragma unroll
       for (int e = 0; e < (D - 1) * P; ++e) wl[e * RL::WS + lane] = sr.w[P + e];
 #pragma unroll
-      for (int c = 0; c < RL::NS; ++c) vst[c * RL::WS + lane] = sr.v[c];
+      for (int c = 0; c < RL::NC; ++c) vst[c * RL::WS + lane] = sr.v[c];
     }
     __syncthreads();
     // issue the next chunk's gathers; they land while this chunk is accumulated
@@ -576,11 +594,11 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
         retire_planes<D, P, T>(lo, b0, nc, lds2, dst);
         lo = b0;
       }
-      read_run<D, P, T>(w0, rok, roff, sgrid, pre);
+      read_run<D, P, T>(w0, rok[0], roff, sgrid, pre);
     }
     // ring of LA register sets, loaded LA-2..LA-1 samples ahead; the stage arrays are
     // padded so loads past the last sample stay in bounds (their values are unused)
-    SampleView<D, P, RL::NR> ring[RL::LA];
+    SampleView<D, P, RL::NR, RL::CPL> ring[RL::LA];
 #pragma unroll
     for (int u = 0; u < RL::LA - 1; ++u) read_view<D, P, T>(ring[u], u, rec, wl, vmine, ridx);
     for (int t = 0; t < n; t += RL::LA) {
@@ -969,7 +987,7 @@ static int launch_op(const LaunchArgs& a) {
     case Op::GridPartial: {
       if (a.nitems <= 0) break;
       using RL = RowLanes<D, P, T>;
-      const int nch = (a.nbatch + RL::NS - 1) / RL::NS;
+      const int nch = (a.nbatch + RL::NC - 1) / RL::NC;
       const size_t sh = (size_t)RL::LDS_FLOAT2 * sizeof(float2);
       grid_partial_kernel<D, P, T><<<item_grid(a.nitems, nch), 64, sh, a.stream>>>(
           (const float2*)a.src, (const float*)a.aux, a.wsort, a.lbase, a.perm, a.geoms, a.items, a.nitems, nch,
