@@ -119,6 +119,24 @@ int mra_fft_lds_supported(int ndim, const int* n);
 int mra_fft_lds_twiddles(int L, float* tw_host);
 int mra_fft_lds(int ndim, const int* n, long long batch, void* data, const void* const* tw, int inverse,
                 void* stream);
+/* FFTs fused with the pad / crop around them, skipping the known zeros of the padded
+ * grid (fourier.py:432-468 forward_FT_only, 505-525 adjoint_iFT_only, 302-352
+ * normal_toeplitz; SENSE expand / reduce linops.py:1161, 1220, 1265-1306).
+ * mra_fft_pad_forward: grid[q] = FFT(pad(x[img(q)] * S[map(q)] * apod * scale));
+ * mra_fft_crop_adjoint: out[i] (+)= scale * apod * sum_q crop(iFFT(grid[q])) * (conj) S[q]
+ * (the grids are overwritten); mra_toeplitz_fused: the PSF normal operator of one image,
+ * work = ns grids. Same grid shapes as mra_fft_lds. */
+int mra_fft_fused_supported(int ndim, const int* G);
+int mra_fft_pad_forward(int ndim, const int* N, const int* G, const int* center, const void* x, long long x_stride,
+                        const void* smaps, long long ns, long long s_stride, const int* img_index, long long nq,
+                        const float* apod, float scale, void* grid, const void* const* tw, void* stream);
+int mra_fft_crop_adjoint(int ndim, const int* N, const int* G, const int* center, void* grid, long long nq,
+                         long long nx, const void* smaps, long long ns, long long s_stride, const int* coil_off,
+                         int conj_maps, const float* apod, float scale, int accumulate, void* out,
+                         long long x_stride, const void* const* tw, void* stream);
+int mra_toeplitz_fused(int ndim, const int* N, const int* G, const int* center, const void* x, const void* smaps,
+                       long long ns, long long s_stride, const void* kern, void* work, int accumulate, void* out,
+                       const void* const* tw, void* stream);
 
 /* ---- conjugate gradient -----------------------------------------------------------------
  * Vector work of conjugate_gradient (algs.py:779-872) with FP64 inner products

@@ -59,6 +59,12 @@ SIGNATURES = {
     "mra_fft_lds_supported": (_i, [_i, _ip]),
     "mra_fft_lds_twiddles": (_i, [_i, _fp]),
     "mra_fft_lds": (_i, [_i, _ip, _ll, _vp, C.POINTER(_vp), _i, _vp]),
+    "mra_fft_fused_supported": (_i, [_i, _ip]),
+    "mra_fft_pad_forward": (_i, [_i, _ip, _ip, _ip, _vp, _ll, _vp, _ll, _ll, _vp, _ll, _vp, _f, _vp, C.POINTER(_vp),
+                                 _vp]),
+    "mra_fft_crop_adjoint": (_i, [_i, _ip, _ip, _ip, _vp, _ll, _ll, _vp, _ll, _ll, _vp, _i, _vp, _f, _i, _vp, _ll,
+                                  C.POINTER(_vp), _vp]),
+    "mra_toeplitz_fused": (_i, [_i, _ip, _ip, _ip, _vp, _vp, _ll, _ll, _vp, _vp, _i, _vp, C.POINTER(_vp), _vp]),
     "mra_cg_partials": (_i, []),
     "mra_cg_seg_blocks": (_i, []),
     "mra_cdot_re": (_i, [_vp, _vp, _ll, _f, _vp, _vp, _i, _vp]),

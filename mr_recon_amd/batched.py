@@ -32,7 +32,7 @@ from . import _lib, ops
 from ._lib import check, lib, ptr, stream_ptr
 from .dtypes import complex_dtype
 from .fourier import _fftn, _ifftn, hip_nufft
-from .plan import SamplePlan, make_geoms
+from .plan import make_geoms
 
 
 @dataclass
@@ -112,14 +112,19 @@ class _Group:
         self.geoms = make_geoms(entries, dev)
         self.scratch = torch.empty(max(1, self.scratch_off[-1]), dtype=complex_dtype, device=dev)
         self.ksp_len = self.ksp_off[-1]
+        self.fused = ops.fused_supported(self.G)  # pad+FFT / iFFT+crop in the transform passes
 
     # -- operators on the stacked images X (nimg, N, N) and k-space Y (sum C_i M_i) --------
     def forward(self, X: torch.Tensor) -> torch.Tensor:
-        g = torch.empty((self.ncoil, *self.G), dtype=complex_dtype, device=self.device)
-        check(lib().mra_pad_expand_batched(2, _lib.ints(self.im), _lib.ints(self.G), None, ptr(X), self.nimg,
-                                           ptr(self.mps), self.ncoil, ptr(self.img_index), ptr(self.apod),
-                                           self.s_fwd, ptr(g), stream_ptr(self.device)))
-        g = _fftn(g, 2)
+        if self.fused:
+            g = ops.pad_fft(X, self.im, self.G, smaps=self.mps, apod=self.apod, scale=self.s_fwd,
+                            img_index=self.img_index, nq=self.ncoil)
+        else:
+            g = torch.empty((self.ncoil, *self.G), dtype=complex_dtype, device=self.device)
+            check(lib().mra_pad_expand_batched(2, _lib.ints(self.im), _lib.ints(self.G), None, ptr(X), self.nimg,
+                                               ptr(self.mps), self.ncoil, ptr(self.img_index), ptr(self.apod),
+                                               self.s_fwd, ptr(g), stream_ptr(self.device)))
+            g = _fftn(g, 2)
         Y = torch.empty(self.ksp_len, dtype=complex_dtype, device=self.device)
         check(lib().mra_interp(2, self.P, ptr(g), ptr(Y), ptr(self.wsort), ptr(self.lbase), ptr(self.perm),
                                ptr(self.geoms), ptr(self.items), self.nitems, self.cc, self.nchunks,
@@ -134,8 +139,12 @@ class _Group:
                                      ptr(self.scratch), st))
         check(lib().mra_grid_reduce(2, self.P, ptr(g), ptr(self.scratch), ptr(self.geoms), 0, self.nimg,
                                     ptr(self.tile_items), self.ntiles, self.maxC, 0, st))
-        g = _ifftn(g, 2)
         X = torch.empty((self.nimg, *self.im), dtype=complex_dtype, device=self.device)
+        if self.fused:
+            ops.ifft_crop(g, self.im, self.G, self.nimg, smaps=self.mps, apod=self.apod, scale=self.s_fwd * scale,
+                          out=X, coil_off=self.coil_off)
+            return X
+        g = _ifftn(g, 2)
         check(lib().mra_crop_reduce_batched(2, _lib.ints(self.im), _lib.ints(self.G), None, ptr(g), self.nimg,
                                             ptr(self.mps), ptr(self.coil_off), ptr(self.apod), self.s_fwd * scale,
                                             ptr(X), st))

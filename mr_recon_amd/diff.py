@@ -33,7 +33,7 @@ from torch.autograd import Function
 
 from . import _lib, ops
 from .dtypes import complex_dtype
-from .fourier import _fftn, _ifftn, hip_nufft
+from .fourier import hip_nufft
 
 
 class _Ctx:
@@ -60,8 +60,8 @@ class _Ctx:
     def grids(self, x: torch.Tensor) -> torch.Tensor:
         """FFT(pad(apod * S_c x)) / sqrt(prod N): (C, *G) in the unshifted layout."""
         x = x.to(complex_dtype).reshape(1, *self.im_size)
-        g = ops.pad_expand(x, self.im_size, self.G, smaps=self.mps, apod=self.apod, scale=self.s_fwd)
-        return _fftn(g.reshape(self.C, *self.G), self.d)
+        g = ops.pad_fft(x, self.im_size, self.G, smaps=self.mps, apod=self.apod, scale=self.s_fwd)
+        return g.reshape(self.C, *self.G)
 
     def forward(self, x):
         out = torch.empty((self.C, self.plan.npts), dtype=complex_dtype, device=x.device)
@@ -70,8 +70,7 @@ class _Ctx:
     def adjoint(self, y, dcf):
         g = torch.empty((self.C, *self.G), dtype=complex_dtype, device=y.device)
         ops.gridding(y.to(complex_dtype).reshape(self.C, -1), self.plan, g, dcf=dcf)
-        g = _ifftn(g, self.d)
-        return ops.crop_reduce(g, self.im_size, self.G, 1, smaps=self.mps, apod=self.apod, scale=self.s_adj)[0]
+        return ops.ifft_crop(g, self.im_size, self.G, 1, smaps=self.mps, apod=self.apod, scale=self.s_adj)[0]
 
     def dk(self, grids, a):
         """dL/dk (M, d) from the KB-derivative gather of ``grids`` against coefficients ``a``."""

@@ -19,9 +19,7 @@ the intermediate grid so they compose with code written against it.
 
 from __future__ import annotations
 
-import ctypes
 import math
-import threading
 from collections import OrderedDict
 from math import ceil
 from typing import Optional
@@ -32,8 +30,8 @@ import torch.fft as fft_torch
 import torch.nn as nn
 
 from . import _lib, ops
-from ._lib import check, lib, ptr, stream_ptr
 from .dtypes import complex_dtype, real_dtype
+from .gridfft import _fftn, _ifftn, _lds_fft  # noqa: F401  (in-place grid FFTs)
 from .pad import PadLast
 from .plan import SamplePlan
 from .utils import batch_iterator, resize
@@ -75,91 +73,6 @@ def calc_toep_kernel_helper(nufft_adj_os: callable, trj: torch.Tensor, weights: 
     return fft(img, dim=tuple(range(-d, 0)))
 
 
-class _FFTPlans:
-    """hipFFT plans for in-place batched C2C transforms, per (device, batch, grid shape)."""
-
-    def __init__(self):
-        self._plans: dict = {}
-        self._lock = threading.Lock()
-
-    def get(self, dev_index: int, batch: int, shape: tuple):
-        key = (dev_index, batch, shape)
-        with self._lock:
-            h = self._plans.get(key)
-            if h is None:
-                h = ctypes.c_void_p()
-                with torch.cuda.device(dev_index):
-                    check(lib().mra_fft_plan(len(shape), _lib.ints(shape), batch, ctypes.byref(h)))
-                self._plans[key] = h
-            return h
-
-
-_fft_plans = _FFTPlans()
-
-
-class _LdsFFT:
-    """Twiddle tables (device) and shape support of the LDS-staged transforms."""
-
-    def __init__(self):
-        self._tw: dict = {}
-        self._ok: dict = {}
-        self._lock = threading.Lock()
-
-    def supported(self, shape: tuple) -> bool:
-        ok = self._ok.get(shape)
-        if ok is None:
-            ok = bool(lib().mra_fft_lds_supported(len(shape), _lib.ints(shape)))
-            self._ok[shape] = ok
-        return ok
-
-    def tables(self, device: torch.device, shape: tuple):
-        key = (device.index, shape)
-        with self._lock:
-            arr = self._tw.get(key)
-            if arr is None:
-                tabs = []
-                for L in shape:
-                    host = torch.empty(2 * L, dtype=torch.float32)
-                    check(lib().mra_fft_lds_twiddles(L, ctypes.cast(host.data_ptr(), ctypes.POINTER(ctypes.c_float))))
-                    tabs.append(host.to(device))
-                arr = (tabs, (ctypes.c_void_p * len(shape))(*[t.data_ptr() for t in tabs]))
-                self._tw[key] = arr
-            return arr[1]
-
-
-_lds_fft = _LdsFFT()
-
-
-def _fft_inplace(x: torch.Tensor, d: int, inverse: bool) -> torch.Tensor:
-    _lib.require_device(x)
-    if x.dtype != torch.complex64 or not x.is_contiguous():
-        raise TypeError("grid FFTs run in place on contiguous complex64 tensors")
-    shape = tuple(int(s) for s in x.shape[-d:])
-    batch = int(math.prod(x.shape[:-d]))
-    if batch == 0:
-        return x
-    if _lds_fft.supported(shape):
-        tw = _lds_fft.tables(x.device, shape)
-        check(lib().mra_fft_lds(d, _lib.ints(shape), batch, ptr(x), tw, int(inverse), stream_ptr(x.device)))
-        return x
-    plan = _fft_plans.get(x.device.index, batch, shape)
-    check(lib().mra_fft_exec(plan, ptr(x), int(inverse), stream_ptr(x.device)))
-    return x
-
-
-def _fftn(x: torch.Tensor, d: int) -> torch.Tensor:
-    """Unnormalised forward FFT over the last d dims, IN PLACE on the caller's temporary
-    grid ``x`` (returned): one LDS-staged pass per axis for lengths with a compiled radix
-    plan, rocFFT (hipFFT) otherwise. An out-of-place transform that preserves its input
-    costs a full copy of the grid first."""
-    return _fft_inplace(x, d, False)
-
-
-def _ifftn(x: torch.Tensor, d: int) -> torch.Tensor:
-    """Unnormalised inverse FFT (no 1/n) over the last d dims, in place on ``x``."""
-    return _fft_inplace(x, d, True)
-
-
 class NUFFT(nn.Module):
     """Forward NUFFT e^{-j 2 pi k . r}, k_i in [-N_i/2, N_i/2], r_i in [-1/2, 1/2] (fourier.py:107-352)."""
 
@@ -195,8 +108,8 @@ class NUFFT(nn.Module):
         """A^H A img through the PSF kernels (reference fourier.py:302-352).
 
         Same composition as the reference (PadLast pad, centred ortho FFT,
-        multiply, centred ortho iFFT, crop), run as pad kernel -> in-place FFT ->
-        multiply kernel -> in-place iFFT -> crop kernel in the unshifted layout.
+        multiply, centred ortho iFFT, crop), run as fused pad+FFT -> multiply
+        kernel -> fused iFFT+crop in the unshifted layout.
         """
         _lib.require_device(img, toeplitz_kernels)
         N = img.shape[0]
@@ -210,11 +123,9 @@ class NUFFT(nn.Module):
         for n in range(N):
             kern = toeplitz_unshifted(toeplitz_kernels[n])
             x = img[n].reshape(B, *self.im_size).to(complex_dtype)
-            g = ops.pad_expand(x, self.im_size, im_size_os, center=cen)
-            g = _fftn(g, d)
+            g = ops.pad_fft(x, self.im_size, im_size_os, center=cen)
             ops.cmul_bcast(g, kern)
-            g = _ifftn(g, d)
-            ops.crop_reduce(g, self.im_size, im_size_os, B, center=cen, out=out[n])
+            ops.ifft_crop(g, self.im_size, im_size_os, B, center=cen, out=out[n])
         return out.reshape(N, *img_batch, *self.im_size)
 
 
@@ -350,8 +261,7 @@ class hip_nufft(NUFFT):
         G = self.os_shape(im)
         lead = img.shape[:-d]
         x = img.reshape(-1, *im).to(complex_dtype)
-        g = ops.pad_expand(x, im, G, apod=self.apod_table(img.device, im), scale=1.0 / math.sqrt(math.prod(im)))
-        g = _fftn(g, d)
+        g = ops.pad_fft(x, im, G, apod=self.apod_table(img.device, im), scale=1.0 / math.sqrt(math.prod(im)))
         g = fft_torch.fftshift(g, dim=tuple(range(-d, 0)))
         return g.reshape(*lead, *G)
 
@@ -396,9 +306,9 @@ class hip_nufft(NUFFT):
         G = tuple(ksp_os.shape[-d:])
         lead = ksp_os.shape[:-d]
         g = fft_torch.ifftshift(ksp_os.to(complex_dtype), dim=tuple(range(-d, 0))).contiguous()
-        g = _ifftn(g.reshape(-1, *G), d)
-        img = ops.crop_reduce(g, self.im_size, G, g.shape[0], apod=self.apod_table(ksp_os.device),
-                              scale=1.0 / math.sqrt(math.prod(self.im_size)))
+        g = g.reshape(-1, *G)
+        img = ops.ifft_crop(g, self.im_size, G, g.shape[0], apod=self.apod_table(ksp_os.device),
+                            scale=1.0 / math.sqrt(math.prod(self.im_size)))
         return img.reshape(*lead, *self.im_size)
 
     # -- fused transforms ------------------------------------------------
@@ -417,9 +327,8 @@ class hip_nufft(NUFFT):
         x = img.to(complex_dtype).reshape(img.shape[0], B, *self.im_size)
         for n in range(N):
             plan = self.plan(trj[n])
-            g = ops.pad_expand(x[n if x.shape[0] > 1 else 0], self.im_size, G, apod=apod,
-                               scale=1.0 / math.sqrt(math.prod(self.im_size)))
-            g = _fftn(g, d)
+            g = ops.pad_fft(x[n if x.shape[0] > 1 else 0], self.im_size, G, apod=apod,
+                            scale=1.0 / math.sqrt(math.prod(self.im_size)))
             ops.interp(g, plan, out_scale=1.0 / self.width ** d, out=out[n])
         return out.reshape(N, *img_batch, *trj_batch)
 
@@ -441,8 +350,7 @@ class hip_nufft(NUFFT):
             plan = self.plan(trj[n])
             g = torch.empty((B, *G), dtype=complex_dtype, device=ksp.device)
             ops.gridding(y[n if y.shape[0] > 1 else 0], plan, g)
-            g = _ifftn(g, d)
-            ops.crop_reduce(g, self.im_size, G, B, apod=apod, scale=scale, out=out[n])
+            ops.ifft_crop(g, self.im_size, G, B, apod=apod, scale=scale, out=out[n])
         return out.reshape(N, *ksp_batch, *self.im_size)
 
     def calc_teoplitz_kernels(self, trj: torch.Tensor, weights: Optional[torch.Tensor] = None,

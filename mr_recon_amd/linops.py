@@ -26,7 +26,7 @@ import torch.nn as nn
 
 from . import _lib, ops
 from .dtypes import complex_dtype, real_dtype
-from .fourier import NUFFT, _fftn, _ifftn, hip_nufft, toeplitz_unshifted
+from .fourier import NUFFT, hip_nufft, toeplitz_unshifted
 from .pad import PadLast
 from .utils import batch_iterator
 
@@ -154,16 +154,14 @@ class sense_linop(linop):
         """img (*im_size) -> ksp (C, *trj_size)."""
         if not self.fused:
             return self._forward_generic(img)
-        d = len(self.im_size)
         C = self.mps.shape[0]
         M = self.plan.npts
         ksp = torch.empty((C, M), dtype=complex_dtype, device=self.torch_dev)
         x = img.to(complex_dtype).reshape(1, *self.im_size)
         for c0, c1 in self._coil_chunks():
-            g = self._mark("pad_expand", ops.pad_expand, x, self.im_size, self.os_shape, smaps=self.mps[c0:c1],
+            g = self._mark("pad_fft", ops.pad_fft, x, self.im_size, self.os_shape, smaps=self.mps[c0:c1],
                            apod=self.apod, scale=self._fwd_scale)
             g = g.view(c1 - c0, *self.os_shape)
-            g = self._mark("fft", _fftn, g, d)
             self._mark("interp", ops.interp, g, self.plan, out_scale=self._interp_scale, out=ksp[c0:c1])
         return ksp.reshape(C, *self.trj_size)
 
@@ -171,7 +169,6 @@ class sense_linop(linop):
         """ksp (C, *trj_size) -> img (*im_size), with the DCF applied to ksp first."""
         if not self.fused:
             return self._adjoint_generic(ksp)
-        d = len(self.im_size)
         C = self.mps.shape[0]
         y = ksp.to(complex_dtype).reshape(C, -1)
         img = torch.empty((1, *self.im_size), dtype=complex_dtype, device=self.torch_dev)
@@ -181,8 +178,7 @@ class sense_linop(linop):
                             dcf=self.dcf.reshape(-1))
             self._mark("grid_reduce", ops.gridding_reduce, sc, self.plan, g)
             del sc
-            g = self._mark("ifft", _ifftn, g, d)
-            self._mark("crop_reduce", ops.crop_reduce, g, self.im_size, self.os_shape, 1, smaps=self.mps[c0:c1],
+            self._mark("ifft_crop", ops.ifft_crop, g, self.im_size, self.os_shape, 1, smaps=self.mps[c0:c1],
                        apod=self.apod, scale=self._adj_scale, out=img, accumulate=i > 0)
         return img[0]
 
@@ -198,11 +194,8 @@ class sense_linop(linop):
         out = torch.empty((1, *self.im_size), dtype=complex_dtype, device=self.torch_dev)
         cen = self.padder.center
         for i, (c0, c1) in enumerate(self._coil_chunks()):
-            g = ops.pad_expand(x, self.im_size, P, smaps=self.mps[c0:c1], center=cen)
-            g = _fftn(g.view(c1 - c0, *P), d)
-            ops.cmul_bcast(g, self.toep_unshifted)
-            g = _ifftn(g, d)
-            ops.crop_reduce(g, self.im_size, P, 1, smaps=self.mps[c0:c1], center=cen, out=out, accumulate=i > 0)
+            ops.toeplitz_normal(x, self.im_size, P, self.mps[c0:c1], self.toep_unshifted, center=cen, out=out,
+                                accumulate=i > 0)
         return out[0]
 
     # -- reference loops (any NUFFT, L >= 1 segments): linops.py:1134-1308 ------------

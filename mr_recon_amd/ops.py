@@ -194,6 +194,150 @@ def crop_reduce(grid: torch.Tensor, im_size, grid_size, nx: int, smaps: torch.Te
     return out
 
 
+_fused_ok: dict = {}
+
+
+def fused_supported(grid_size) -> bool:
+    """Whether the fused pad+FFT / iFFT+crop transforms cover grids of this shape."""
+    key = tuple(int(g) for g in grid_size)
+    ok = _fused_ok.get(key)
+    if ok is None:
+        ok = bool(lib().mra_fft_fused_supported(len(key), _lib.ints(key)))
+        _fused_ok[key] = ok
+    return ok
+
+
+def pad_fft(x: torch.Tensor, im_size, grid_size, smaps: torch.Tensor | None = None,
+            apod: torch.Tensor | None = None, scale: float = 1.0, center=None,
+            img_index: torch.Tensor | None = None, nq: int | None = None) -> torch.Tensor:
+    """Spectrum of the zero-padded images: FFT(pad_expand(x, smaps, apod, scale)) as a
+    new grid tensor (nx[, C], *G), unnormalised, unshifted layout. With ``img_index``
+    (batched problems), grid q pads image img_index[q] times map q (nq grids)."""
+    from .gridfft import _fftn, _lds_fft
+
+    D = len(im_size)
+    G = tuple(int(g) for g in grid_size)
+    if not fused_supported(G):
+        if img_index is not None:
+            raise ValueError("pad_fft: batched padding needs a grid with an LDS FFT plan")
+        g = pad_expand(x, im_size, G, smaps=smaps, apod=apod, scale=scale, center=center)
+        return _fftn(g.reshape(-1, *G), D).reshape(g.shape)
+    _lib.require_device(x, smaps, apod, img_index)
+    x = _c64(x, "x")
+    if tuple(x.shape[-D:]) != tuple(im_size):
+        raise ValueError(f"image shape {tuple(x.shape[-D:])} != {tuple(im_size)}")
+    nx = int(math.prod(x.shape[:-D]))
+    npix = int(math.prod(im_size))
+    ns = 1
+    if smaps is not None:
+        smaps = _c64(smaps, "smaps")
+        if tuple(smaps.shape[-D:]) != tuple(im_size):
+            raise ValueError("maps must have the image shape")
+        ns = int(math.prod(smaps.shape[:-D]))
+    if apod is not None and (apod.dtype != torch.float32 or apod.numel() != sum(im_size)):
+        raise ValueError("apod must be float32 with sum(im_size) entries")
+    if img_index is not None:
+        q = int(nq)
+        if img_index.dtype != torch.int32 or img_index.numel() != q:
+            raise ValueError("img_index must be int32 with one entry per grid")
+        if smaps is not None and ns < q:
+            raise ValueError("batched padding needs one map per grid")
+        shape = (q, *G)
+    else:
+        q = nx * ns
+        shape = (nx, ns, *G) if smaps is not None else (nx, *G)
+    out = torch.empty(shape, dtype=torch.complex64, device=x.device)
+    cen = None if center is None else _lib.ints(center)
+    check(lib().mra_fft_pad_forward(D, _lib.ints(im_size), _lib.ints(G), cen, ptr(x), npix, ptr(smaps), ns, npix,
+                                    ptr(img_index), q, ptr(apod), float(scale), ptr(out),
+                                    _lds_fft.tables(x.device, G), stream_ptr(x.device)))
+    return out
+
+
+def ifft_crop(grid: torch.Tensor, im_size, grid_size, nx: int, smaps: torch.Tensor | None = None,
+              apod: torch.Tensor | None = None, scale: float = 1.0, out: torch.Tensor | None = None,
+              accumulate: bool = False, conj_maps: bool = True, center=None,
+              coil_off: torch.Tensor | None = None) -> torch.Tensor:
+    """crop_reduce(iFFT(grid)) (unnormalised inverse) into image (nx, *N). The grid is
+    used as work space and overwritten. With ``coil_off`` (batched problems), image i
+    reduces grids and maps coil_off[i] .. coil_off[i+1]-1."""
+    from .gridfft import _ifftn, _lds_fft
+
+    D = len(im_size)
+    G = tuple(int(g) for g in grid_size)
+    if not fused_supported(G):
+        if coil_off is not None:
+            raise ValueError("ifft_crop: batched reduction needs a grid with an LDS FFT plan")
+        g = _ifftn(grid.reshape(-1, *G), D)
+        return crop_reduce(g, im_size, G, nx, smaps=smaps, apod=apod, scale=scale, out=out,
+                           accumulate=accumulate, conj_maps=conj_maps, center=center)
+    _lib.require_device(grid, smaps, apod, out, coil_off)
+    if grid.dtype != torch.complex64 or not grid.is_contiguous():
+        raise TypeError("ifft_crop: grid must be contiguous complex64 (it is overwritten)")
+    npix = int(math.prod(im_size))
+    nq = grid.numel() // math.prod(G)
+    ns = 1
+    if smaps is not None:
+        smaps = _c64(smaps, "smaps")
+        ns = int(math.prod(smaps.shape[:-D]))
+    if coil_off is None and nq != nx * ns:
+        raise ValueError(f"ifft_crop: {nq} grids for {nx} images x {ns} maps")
+    if coil_off is not None:
+        if coil_off.dtype != torch.int32 or coil_off.numel() != nx + 1:
+            raise ValueError("coil_off must be int32 with nx + 1 entries")
+        if smaps is not None and ns < nq:
+            raise ValueError("batched reduction needs one map per grid")
+    if apod is not None and (apod.dtype != torch.float32 or apod.numel() != sum(im_size)):
+        raise ValueError("apod must be float32 with sum(im_size) entries")
+    if out is None:
+        if accumulate:
+            raise ValueError("accumulate needs an output tensor")
+        out = torch.empty((nx, *im_size), dtype=torch.complex64, device=grid.device)
+    elif out.numel() != nx * npix or out.dtype != torch.complex64 or not out.is_contiguous():
+        raise ValueError("ifft_crop: bad output tensor")
+    cen = None if center is None else _lib.ints(center)
+    check(lib().mra_fft_crop_adjoint(D, _lib.ints(im_size), _lib.ints(G), cen, ptr(grid), nq, nx, ptr(smaps), ns,
+                                     npix, ptr(coil_off), int(conj_maps), ptr(apod), float(scale), int(accumulate),
+                                     ptr(out), npix, _lds_fft.tables(grid.device, G), stream_ptr(grid.device)))
+    return out
+
+
+def toeplitz_normal(x: torch.Tensor, im_size, grid_size, smaps: torch.Tensor, kern_unshifted: torch.Tensor,
+                    center=None, out: torch.Tensor | None = None, accumulate: bool = False) -> torch.Tensor:
+    """sum_c conj(S_c) crop(iFFT(K * FFT(pad(x S_c)))) for one image x (*N) and maps (C, *N),
+    K the PSF multiplier on grid ``grid_size`` (unshifted layout, normalisation included)."""
+    from .gridfft import _fftn, _ifftn, _lds_fft
+
+    D = len(im_size)
+    G = tuple(int(g) for g in grid_size)
+    x = _c64(x, "x").reshape(1, *im_size)
+    smaps = _c64(smaps, "smaps")
+    C = int(math.prod(smaps.shape[:-D]))
+    if D < 2 or not fused_supported(G):
+        g = pad_expand(x, im_size, G, smaps=smaps, center=center)
+        g = _fftn(g.reshape(C, *G), D)
+        cmul_bcast(g, kern_unshifted)
+        g = _ifftn(g, D)
+        return crop_reduce(g, im_size, G, 1, smaps=smaps, center=center, out=out, accumulate=accumulate)
+    _lib.require_device(x, smaps, kern_unshifted, out)
+    kern = _c64(kern_unshifted, "kern")
+    if tuple(kern.shape[-D:]) != G:
+        raise ValueError("toeplitz kernel shape != grid shape")
+    npix = int(math.prod(im_size))
+    if out is None:
+        if accumulate:
+            raise ValueError("accumulate needs an output tensor")
+        out = torch.empty((1, *im_size), dtype=torch.complex64, device=x.device)
+    elif out.numel() != npix or out.dtype != torch.complex64 or not out.is_contiguous():
+        raise ValueError("toeplitz_normal: bad output tensor")
+    work = torch.empty((C, *G), dtype=torch.complex64, device=x.device)
+    cen = None if center is None else _lib.ints(center)
+    check(lib().mra_toeplitz_fused(D, _lib.ints(im_size), _lib.ints(G), cen, ptr(x), ptr(smaps), C, npix, ptr(kern),
+                                   ptr(work), int(accumulate), ptr(out), _lds_fft.tables(x.device, G),
+                                   stream_ptr(x.device)))
+    return out
+
+
 def cmul_bcast(data: torch.Tensor, kern: torch.Tensor) -> torch.Tensor:
     """In place: data (B, *S) *= kern (*S), complex64."""
     _lib.require_device(data, kern)
