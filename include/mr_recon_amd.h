@@ -73,6 +73,11 @@ int mra_grid_partial(int ndim, int P, const void* ksp, const float* dcf, const f
                      void* scratch, void* stream);
 int mra_grid_reduce(int ndim, int P, void* grid, const void* scratch, const void* geoms, int prob0, int nprob,
                     const int* tile_items, int ntiles, int nbatch, int accumulate, void* stream);
+/* Selects the phase-1 kernel behind mra_grid_partial: 1 (default) = 3D cells of samples
+ * summed on the matrix cores (P <= 7), 0 = per-lane LDS read-modify-write. Same scratch
+ * output; returns the previous selection. No reference counterpart (one kernel per
+ * dimensionality in grid.py:150-502). */
+int mra_set_grid_kernel(int mode);
 /* Gradient of Re<a, interp(grid, t)> with respect to the sample coordinates t.
  * No reference counterpart: InterpolateFn.backward returns None for locs
  * (triton_interp/interp.py:66-74). */

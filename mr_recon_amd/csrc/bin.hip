@@ -36,6 +36,13 @@ struct BinParams {
   int ntiles;             // product of ntile; key ntiles = non-finite sample
 };
 
+// Position of a corner (a0, a1, a2) inside a 3D tile of edge T (a multiple of 4), ordered
+// by cell (a0, a1 / 2, a2 / 4) first, then (a1 % 2, a2 % 4): the samples of one 1x2x4 cell
+// of corners are contiguous and a0 never decreases along a tile.
+__device__ __forceinline__ int cell_major_local(int a0, int a1, int a2, int T) {
+  return (((a0 * (T / 2) + a1 / 2) * (T / 4) + a2 / 4) * 2 + a1 % 2) * 4 + a2 % 4;
+}
+
 // Scaled, wrapped coordinates of sample i, its tile id and corner position in the tile.
 template <int D>
 __device__ __forceinline__ int sample_key(const float* __restrict__ coords, long long i, const BinParams& bp,
@@ -45,6 +52,7 @@ __device__ __forceinline__ int sample_key(const float* __restrict__ coords, long
   // multiply the backend would otherwise fuse with the add into an FMA).
   bool ok = true;
   int key = 0, loc = 0;
+  int a[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     float v = coords[i * D + d];
@@ -60,8 +68,10 @@ __device__ __forceinline__ int sample_key(const float* __restrict__ coords, long
     const int lower = ok ? (int)ceilf(__fsub_rn(m, bp.r)) : 0;
     const int lw = pmod(lower + bp.wrap[d], bp.G[d]);
     key = key * bp.ntile[d] + lw / bp.T;
-    loc = loc * bp.T + lw % bp.T;
+    a[d] = lw % bp.T;
+    loc = loc * bp.T + a[d];
   }
+  if constexpr (D == 3) loc = cell_major_local(a[0], a[1], a[2], bp.T);
   local = ok ? loc : 0;
   return ok ? key : bp.ntiles;
 }

@@ -615,6 +615,12 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
   retire_planes<D, P, T>(lo, SUB, nc, lds2, dst);
 }
 
+}  // namespace mra
+
+#include "grid_mfma.h"
+
+namespace mra {
+
 // Gridding phase 2. One block per (problem, output tile, group of RB batch entries):
 // sums, in a fixed order, the scratch subgrids of every item whose subgrid covers the
 // tile and writes each grid point once. blockIdx.x = (prob - prob0) * ntiles + tile,
@@ -966,6 +972,9 @@ struct LaunchArgs {
 };
 
 
+// Phase-1 gridding kernel: 1 = matrix cores (3D, P <= 7), 0 = per-lane LDS read-modify-write.
+static int g_grid_kernel = 1;
+
 template <int D, int P>
 static int launch_op(const LaunchArgs& a) {
   constexpr int T = TileCfg<D>::T;
@@ -986,6 +995,16 @@ static int launch_op(const LaunchArgs& a) {
     }
     case Op::GridPartial: {
       if (a.nitems <= 0) break;
+      if constexpr (D == 3 && P >= 2 && P <= 7) {
+        if (g_grid_kernel == 1) {
+          using M = MfmaGrid<P>;
+          const int ng = (a.nbatch + M::NC - 1) / M::NC;
+          grid_mfma_kernel<P><<<item_grid(a.nitems, ng), 64 * M::NW, 0, a.stream>>>(
+              (const float2*)a.src, (const float*)a.aux, a.wsort, a.lbase, a.perm, a.geoms, a.items, a.nitems, ng,
+              (float2*)a.dst);
+          break;
+        }
+      }
       using RL = RowLanes<D, P, T>;
       const int nch = (a.nbatch + RL::NC - 1) / RL::NC;
       const size_t sh = (size_t)RL::LDS_FLOAT2 * sizeof(float2);
@@ -1060,6 +1079,14 @@ using namespace mra;
 extern "C" {
 
 int mra_geom_size() { return (int)sizeof(Geom); }
+
+// Select the phase-1 gridding kernel (1 = matrix cores where compiled, 0 = per-lane LDS
+// read-modify-write); returns the previous selection. Both write the same scratch.
+int mra_set_grid_kernel(int mode) {
+  const int old = g_grid_kernel;
+  g_grid_kernel = mode ? 1 : 0;
+  return old;
+}
 
 int mra_default_tile(int ndim) {
   switch (ndim) {

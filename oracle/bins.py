@@ -47,8 +47,9 @@ def lower_corner(t: np.ndarray, width: float) -> np.ndarray:
 
 def tile_keys(t: np.ndarray, grid_size, width, tile, wrap, with_local: bool = False):
     """Linear tile id per sample (non-finite samples get ntiles) and, with
-    ``with_local``, the row-major position of the wrapped lower corner inside its
-    tile (0 for non-finite samples)."""
+    ``with_local``, the position of the wrapped lower corner inside its tile (0 for
+    non-finite samples): row-major in 1D/2D; in 3D ordered by 1x2x4 cell of corners
+    first (corner plane, row // 2, column // 4), then by position in the cell."""
     ntile = [(g + tile - 1) // tile for g in grid_size]
     ntiles = math.prod(ntile)
     ok = np.all(np.isfinite(t), axis=-1)
@@ -56,10 +57,15 @@ def tile_keys(t: np.ndarray, grid_size, width, tile, wrap, with_local: bool = Fa
     lo = np.where(np.isfinite(lo), lo, 0).astype(np.int64)
     key = np.zeros(t.shape[0], dtype=np.int64)
     loc = np.zeros(t.shape[0], dtype=np.int64)
+    a = []
     for d, G in enumerate(grid_size):
         lw = np.mod(lo[:, d] + wrap[d], G)
         key = key * ntile[d] + lw // tile
+        a.append(lw % tile)
         loc = loc * tile + lw % tile
+    if len(grid_size) == 3:
+        # cell-major: cell (a0, a1 // 2, a2 // 4), then (a1 % 2, a2 % 4)
+        loc = (((a[0] * (tile // 2) + a[1] // 2) * (tile // 4) + a[2] // 4) * 2 + a[1] % 2) * 4 + a[2] % 4
     key = np.where(ok, key, ntiles)
     if with_local:
         return key, np.where(ok, loc, 0), ntiles

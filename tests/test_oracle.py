@@ -96,6 +96,32 @@ def test_bin_plan_is_stable_sort():
     assert start[-1] == k.shape[0] and start[-2] == k.shape[0] - 1
 
 
+def test_bin_plan_3d_cell_order():
+    """3D: within a tile the order is by corner plane a0, then 1x2x4 cell (a1 // 2, a2 // 4),
+    then position in the cell (a1 % 2, a2 % 4), then sample index -- Python's sort of
+    tuples; every cell's samples are contiguous and a0 never decreases within a tile."""
+    rng = np.random.default_rng(9)
+    G, T = 36, 8
+    k = (rng.random((6000, 3)) * G).astype(np.float32)
+    k[:300] = k[:300] * np.float32(0.2) + np.float32(10)  # crowd a few tiles
+    k[11] = np.inf
+    wrap = (-18, 0, 5)
+    ts, perm, start, key = obins.bin_plan(k, (G,) * 3, 4.0, T, wrap=wrap)
+    t = obins.wrap_coords(k, (G,) * 3)
+    nt = -(-G // T)
+    tuples = []
+    for i in range(k.shape[0]):
+        if not np.all(np.isfinite(t[i])):
+            tuples.append((nt ** 3, 0, 0, 0, 0, 0, i))
+            continue
+        lw = [(math.ceil(np.float32(t[i, d] - np.float32(2.0))) + wrap[d]) % G for d in range(3)]
+        tile = ((lw[0] // T) * nt + lw[1] // T) * nt + lw[2] // T
+        a = [x % T for x in lw]
+        tuples.append((tile, a[0], a[1] // 2, a[2] // 4, a[1] % 2, a[2] % 4, i))
+    want = [x[-1] for x in sorted(tuples)]
+    assert perm.tolist() == want
+
+
 @pytest.mark.parametrize("D,G,w", [(1, 7, 4.0), (2, 6, 4.0), (2, 9, 6.0), (3, 5, 4.0), (2, 8, 3.5)])
 def test_ungrid_vs_bruteforce(D, G, w):
     g = torch.Generator().manual_seed(D + G)
