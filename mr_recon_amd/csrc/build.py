@@ -37,7 +37,10 @@ CXXFLAGS = [
 ]
 # Per-file extra flags. Binning reproduces the reference's separately rounded
 # fp32 multiply and add bit for bit, so no FMA contraction there.
-FILE_FLAGS = {"bin.hip": ["-ffp-contract=off"]}
+# bin.hip: coordinate arithmetic must round like separate fp32 operations.
+# nufft_kernels.hip: matrix-core accumulators in ordinary VGPRs (the gridding kernel reads
+# them with vector ALU ops at every cell end; in AGPRs each group would copy them out).
+FILE_FLAGS = {"bin.hip": ["-ffp-contract=off"], "nufft_kernels.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def hipcc() -> str:

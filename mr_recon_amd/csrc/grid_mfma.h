@@ -32,7 +32,13 @@
 #define MRA_MFMA_NCW 2  // coils per wave (2 or 4)
 #endif
 #ifndef MRA_MFMA_NW
-#define MRA_MFMA_NW 4   // waves per block (1, 2 or 4)
+#define MRA_MFMA_NW 2   // waves per block (1, 2 or 4)
+#endif
+#ifndef MRA_MFMA_E1
+#define MRA_MFMA_E1 2   // cell extent along dim 1 (corner rows: 2, 4 or 8)
+#endif
+#ifndef MRA_MFMA_E2
+#define MRA_MFMA_E2 4   // cell extent along dim 2 (corner positions: 4 or 8)
 #endif
 
 namespace mra {
@@ -50,8 +56,8 @@ struct MfmaGrid {
   static constexpr int NC = NW * NCW;          // coils per block
   static constexpr int PV = 8 / NCW;           // values of p2 per 16 rows
   static constexpr int PF = 2 * NCW;           // floats per grid point in the ring
-  static constexpr int E1 = 2;                 // cell extent along dim 1 (corner rows)
-  static constexpr int E2 = 4;                 // cell extent along dim 2 (corner positions)
+  static constexpr int E1 = MRA_MFMA_E1;       // cell extent along dim 1 (corner rows)
+  static constexpr int E2 = MRA_MFMA_E2;       // cell extent along dim 2 (corner positions)
   static constexpr int E1P = E1 + P - 1;       // region extent along dim 1
   static constexpr int E2P = E2 + P - 1;       // region extent along dim 2
   static constexpr int RB = (E2P + PV - 1) / PV;  // row blocks
@@ -63,13 +69,14 @@ struct MfmaGrid {
   // ring layout (floats): [slot P][p1 SUB][p2 SUB][coil NCW][re, im]; row and plane
   // pitches padded (a bank model of the cell flush: at most 2-way conflicts)
   static constexpr int ROW = SUB * PF + 4;
-  static constexpr int PLANE = SUB * ROW + (NCW == 2 ? 8 : 40);
+  static constexpr int PLANE = SUB * ROW + (NCW == 2 ? (E1 == 4 ? 48 : 8) : 40);
   static constexpr int DUM = 64 * 4 + RB * PV * PF;  // dummy slots of lanes with no result
   static constexpr int RING = P * PLANE + DUM;
   static_assert(NCW == 2 || NCW == 4, "2 or 4 coils per wave");
   static_assert(NW == 1 || NW == 2 || NW == 4, "1, 2 or 4 waves per block");
   static_assert(CPW % 4 == 0, "B columns are staged four at a time");
-  static_assert(T % E1 == 0 && T % E2 == 0, "cells tile the tile");
+  // cells must be contiguous in the plan's order (a0, a1 / 2, a2 / 4, a1 % 2, a2 % 4)
+  static_assert((E1 == 2 && (E2 == 4 || E2 == 8)) || (E2 == 8 && (E1 == 4 || E1 == 8)), "cell shape");
 };
 
 // Planes lo .. hi-1 of this wave's ring are final: store them to scratch (dense) and
@@ -296,25 +303,24 @@ __global__ __launch_bounds__(64 * MfmaGrid<P>::NW) void grid_mfma_kernel(
     const int cs = sb ? 63 - __builtin_clzll(sb) : 0;
     unsigned long long gm = __ballot(lane < n && ((lane - cs) & 3) == 0);
 
-    auto load_ops = [&](int s, int gsz, float (&ao)[RB], float (&bo)[CB]) {
+    // raw operands of the group at s (gsz samples): y, dim-2 weights, B columns
+    float y, cw[RB], bv[CB];
+    auto load_ops = [&](int s, int gsz) {
       const int sk = s + (q < gsz ? q : 0);  // slot q >= gsz: zero A, B of a valid sample
-      const float y = q < gsz ? syw[sk * PF + yi] : 0.0f;
+      y = q < gsz ? syw[sk * PF + yi] : 0.0f;
       const float* cwp = scw + (sk * M::PV + p2s) * M::CWS;
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb) ao[rb] = cwp[rb] * y;
+      for (int rb = 0; rb < RB; ++rb) cw[rb] = cwp[rb];
 #pragma unroll
-      for (int cb = 0; cb < CB; ++cb) bo[cb] = sbw[(cb * CH + sk) * 16 + i16];
+      for (int cb = 0; cb < CB; ++cb) bv[cb] = sbw[(cb * CH + sk) * 16 + i16];
     };
     int s = (int)__builtin_ctzll(gm);  // 0
     gm &= gm - 1;
     int s2 = gm ? (int)__builtin_ctzll(gm) : n;
-    float a[RB], b[CB];
-    load_ops(s, s2 - s, a, b);
+    load_ops(s, s2 - s);
     while (true) {
       const unsigned long long gm2 = gm & (gm - 1);
       const int s3 = gm2 ? (int)__builtin_ctzll(gm2) : n;
-      float a2[RB], b2[CB];
-      if (s2 < n) load_ops(s2, s3 - s2, a2, b2);
       if ((smask >> s) & 1ull) {
         // a new cell; the previous one was written back unless it ran to a stage end
         const int nk = __builtin_amdgcn_readlane(key, s);
@@ -337,27 +343,35 @@ __global__ __launch_bounds__(64 * MfmaGrid<P>::NW) void grid_mfma_kernel(
         pending = true;
       }
       const bool last = s2 < n && ((smask >> s2) & 1ull);  // the cell ends with this group
-      CellFlush<P> f;
+      // products of this group; its operand registers are free once they are issued, so
+      // the next group's operands are read into them right after. Written out for both
+      // paths so that each waits only for its own loads.
+      auto products = [&]() {
+        float av[RB], bb[CB];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb) av[rb] = cw[rb] * y;
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) bb[cb] = bv[cb];
+#pragma unroll
+        for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+          for (int cb = 0; cb < CB; ++cb) acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rb], bb[cb], acc[rb][cb], 0, 0, 0);
+        if (s2 < n) load_ops(s2, s3 - s2);
+      };
       if (last) {
+        CellFlush<P> f;
         f.locate(ring, ckey, j0, loff, prow, lane);
         f.read();
-      }
-#pragma unroll
-      for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-        for (int cb = 0; cb < CB; ++cb) acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
-      if (last) {
+        products();
         f.write(acc);
         pending = false;
+      } else {
+        products();
       }
       if (s2 >= n) break;
       s = s2;
       s2 = s3;
       gm = gm2;
-#pragma unroll
-      for (int rb = 0; rb < RB; ++rb) a[rb] = a2[rb];
-#pragma unroll
-      for (int cb = 0; cb < CB; ++cb) b[cb] = b2[cb];
     }
     __syncthreads();  // the stage is rewritten next
   }
