@@ -40,6 +40,9 @@
 #ifndef MRA_MFMA_E2
 #define MRA_MFMA_E2 4   // cell extent along dim 2 (corner positions: 4 or 8)
 #endif
+#ifndef MRA_MFMA_KS
+#define MRA_MFMA_KS 4   // matrix instructions' K-steps (4 samples each) per control step
+#endif
 
 namespace mra {
 
@@ -66,6 +69,7 @@ struct MfmaGrid {
   static constexpr int CB = (NJ + 15) / 16;    // column blocks
   static constexpr int CPW = CB * 16 / NW;     // B columns staged per wave
   static constexpr int CH = 64;                // samples per stage
+  static constexpr int KS = MRA_MFMA_KS;       // K-steps per control step (up to 4*KS samples of a cell)
   // ring layout (floats): [slot P][p1 SUB][p2 SUB][coil NCW][re, im]; row and plane
   // pitches padded (a bank model of the cell flush: at most 2-way conflicts)
   static constexpr int ROW = SUB * PF + 4;
@@ -221,18 +225,20 @@ __global__ __launch_bounds__(64 * MfmaGrid<P>::NW) void grid_mfma_kernel(
     loff[cb] = (j % M::E1P) * M::ROW + prow * PF + (4 * q) % PF;
   }
 
-  // gathered sample data of the next stage (registers)
+  // gathered sample data of the next stage (registers); the sample indices (perm) are
+  // read one stage earlier still, so no gather waits for a dependent load
   int rbase = 0;
   float rw[3 * P];
   float2 ry[M::NCW];
+  int onext = lane < cnt ? perm[gs0 + lane] : 0;
   auto gather = [&](int s0) {
     const bool ok = s0 + lane < cnt;
     const long long gs = gs0 + (ok ? s0 + lane : 0);
+    const int o = onext;
     rbase = ok ? lbase[gs] : 0;
     const float* wp = wsort + gs * (3 * P);
 #pragma unroll
     for (int e = 0; e < 3 * P; ++e) rw[e] = ok ? wp[e] : 0.0f;
-    const int o = ok ? perm[gs] : 0;
     const float sc = (ok && dcf) ? dcf[g.samp_off + o] * g.out_scale : g.out_scale;
     const float2* kc = ksp + g.ksp_off + (long long)cw0 * g.ksp_stride + o;
 #pragma unroll
@@ -240,6 +246,7 @@ __global__ __launch_bounds__(64 * MfmaGrid<P>::NW) void grid_mfma_kernel(
       const float2 v = (ok && c < nc) ? kc[(long long)c * g.ksp_stride] : make_float2(0.f, 0.f);
       ry[c] = make_float2(v.x * sc, v.y * sc);
     }
+    onext = s0 + CH + lane < cnt ? perm[gs0 + s0 + CH + lane] : 0;
   };
   gather(0);
 
@@ -301,24 +308,35 @@ __global__ __launch_bounds__(64 * MfmaGrid<P>::NW) void grid_mfma_kernel(
     const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1;
     const unsigned long long sb = smask & upto;
     const int cs = sb ? 63 - __builtin_clzll(sb) : 0;
-    unsigned long long gm = __ballot(lane < n && ((lane - cs) & 3) == 0);
+    unsigned long long gm = __ballot(lane < n && ((lane - cs) % (4 * M::KS)) == 0);
 
-    // raw operands of the group at s (gsz samples): y, dim-2 weights, B columns
-    float y, cw[RB], bv[CB];
-    auto load_ops = [&](int s, int gsz) {
-      const int sk = s + (q < gsz ? q : 0);  // slot q >= gsz: zero A, B of a valid sample
-      y = q < gsz ? syw[sk * PF + yi] : 0.0f;
-      const float* cwp = scw + (sk * M::PV + p2s) * M::CWS;
+    // raw operands of a group of gsz <= 4*KS samples at s: per K-step k (samples
+    // s+4k .. s+4k+3) y, dim-2 weights, B columns
+    constexpr int KS = M::KS;
+    struct Ops {
+      float y[KS], cw[KS][RB], bv[KS][CB];
+    };
+    auto load_ops = [&](Ops& o, int s, int gsz) {
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb) cw[rb] = cwp[rb];
+      for (int k = 0; k < KS; ++k) {
+        if (4 * k < gsz) {
+          const int sk = s + 4 * k + (4 * k + q < gsz ? q : 0);  // empty slot: zero A, B of a valid sample
+          o.y[k] = 4 * k + q < gsz ? syw[sk * PF + yi] : 0.0f;
+          const float* cwp = scw + (sk * M::PV + p2s) * M::CWS;
 #pragma unroll
-      for (int cb = 0; cb < CB; ++cb) bv[cb] = sbw[(cb * CH + sk) * 16 + i16];
+          for (int rb = 0; rb < RB; ++rb) o.cw[k][rb] = cwp[rb];
+#pragma unroll
+          for (int cb = 0; cb < CB; ++cb) o.bv[k][cb] = sbw[(cb * CH + sk) * 16 + i16];
+        }
+      }
     };
     int s = (int)__builtin_ctzll(gm);  // 0
     gm &= gm - 1;
     int s2 = gm ? (int)__builtin_ctzll(gm) : n;
-    load_ops(s, s2 - s);
-    while (true) {
+    // One group with operands `cur`; the next group's operands go to `nxt` once this
+    // group's products are issued. Returns false after the chunk's last group. The loop
+    // below alternates two operand sets so that no register copy waits for the loads.
+    auto step = [&](Ops& cur, Ops& nxt) -> bool {
       const unsigned long long gm2 = gm & (gm - 1);
       const int s3 = gm2 ? (int)__builtin_ctzll(gm2) : n;
       if ((smask >> s) & 1ull) {
@@ -343,20 +361,22 @@ __global__ __launch_bounds__(64 * MfmaGrid<P>::NW) void grid_mfma_kernel(
         pending = true;
       }
       const bool last = s2 < n && ((smask >> s2) & 1ull);  // the cell ends with this group
-      // products of this group; its operand registers are free once they are issued, so
-      // the next group's operands are read into them right after. Written out for both
-      // paths so that each waits only for its own loads.
+      // written out for both paths so that each waits only for its own loads
+      const int gsz = s2 - s;
       auto products = [&]() {
-        float av[RB], bb[CB];
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb) av[rb] = cw[rb] * y;
+        for (int k = 0; k < KS; ++k) {
+          if (4 * k < gsz) {
 #pragma unroll
-        for (int cb = 0; cb < CB; ++cb) bb[cb] = bv[cb];
+            for (int rb = 0; rb < RB; ++rb) {
+              const float av = cur.cw[k][rb] * cur.y[k];
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-          for (int cb = 0; cb < CB; ++cb) acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[rb], bb[cb], acc[rb][cb], 0, 0, 0);
-        if (s2 < n) load_ops(s2, s3 - s2);
+              for (int cb = 0; cb < CB; ++cb)
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, cur.bv[k][cb], acc[rb][cb], 0, 0, 0);
+            }
+          }
+        }
+        if (s2 < n) load_ops(nxt, s2, s3 - s2);
       };
       if (last) {
         CellFlush<P> f;
@@ -368,10 +388,15 @@ __global__ __launch_bounds__(64 * MfmaGrid<P>::NW) void grid_mfma_kernel(
       } else {
         products();
       }
-      if (s2 >= n) break;
+      if (s2 >= n) return false;
       s = s2;
       s2 = s3;
       gm = gm2;
+      return true;
+    };
+    Ops opa, opb;
+    load_ops(opa, s, s2 - s);
+    while (step(opa, opb) && step(opb, opa)) {
     }
     __syncthreads();  // the stage is rewritten next
   }
