@@ -56,30 +56,41 @@ __global__ __launch_bounds__(kThreadsFFT) void pad_row_kernel(const float2* __re
   using S = Shape<L, true>;
   __shared__ float2 buf[L * S::NTP];
   __shared__ float2 tw[L];
+  // per row of the block: image / map offsets of its pixels, apodization, grid row offset
+  __shared__ long long rsrc[S::NT], rmap[S::NT], rdst[S::NT];
+  __shared__ float rapo[S::NT];
   const int tid = threadIdx.x;
   for (int m = tid; m < L; m += kThreadsFFT) tw[m] = twg[m];
   const long long row0 = (long long)blockIdx.x * S::NT;
   const int N0 = p.N[0], N1 = p.N[1], N2 = p.N[2];
-  for (int e = tid; e < S::NT * L; e += kThreadsFFT) {
-    const int c = e / L, j = e - c * L;
-    const long long row = row0 + c;
-    float2 v = make_float2(0.f, 0.f);
+  if (tid < S::NT) {
+    const long long row = row0 + tid;
     if (row < nrows) {
       const int n1 = (int)(row % N1);
       const long long r2 = row / N1;
       const int n0 = (int)(r2 % N0);
       const long long q = r2 / N0;
-      const int n2 = pix_of(j, N2, L, p.c[2]);
-      if (n2 >= 0) {
-        const long long bs = img_index ? q : q % ns;
-        const long long bx = img_index ? img_index[q] : q / ns;
-        const long long pix = ((long long)n0 * N1 + n1) * N2 + n2;
-        v = x[bx * x_stride + pix];
-        if (smaps) v = cmul(v, smaps[bs * s_stride + pix]);
-        const float a = p.scale * apod_at(p, 0, n0) * apod_at(p, 1, n1) * apod_at(p, 2, n2);
-        v.x *= a;
-        v.y *= a;
-      }
+      const long long bs = img_index ? q : q % ns;
+      const long long bx = img_index ? img_index[q] : q / ns;
+      const long long pix = ((long long)n0 * N1 + n1) * N2;
+      rsrc[tid] = bx * x_stride + pix;
+      rmap[tid] = bs * s_stride + pix;
+      rapo[tid] = p.scale * apod_at(p, 0, n0) * apod_at(p, 1, n1);
+      rdst[tid] = ((q * p.G[0] + grid_of(n0, p.G[0], p.c[0])) * p.G[1] + grid_of(n1, p.G[1], p.c[1])) * L;
+    }
+  }
+  __syncthreads();
+  const int nr = (int)min((long long)S::NT, nrows - row0);
+  for (int e = tid; e < S::NT * L; e += kThreadsFFT) {
+    const int c = e / L, j = e - c * L;
+    float2 v = make_float2(0.f, 0.f);
+    const int n2 = pix_of(j, N2, L, p.c[2]);
+    if (c < nr && n2 >= 0) {
+      v = x[rsrc[c] + n2];
+      if (smaps) v = cmul(v, smaps[rmap[c] + n2]);
+      const float a = rapo[c] * apod_at(p, 2, n2);
+      v.x *= a;
+      v.y *= a;
     }
     buf[j * S::NTP + c] = v;
   }
@@ -87,15 +98,7 @@ __global__ __launch_bounds__(kThreadsFFT) void pad_row_kernel(const float2* __re
   fft_lds<L, S>(buf, tw);
   for (int e = tid; e < S::NT * L; e += kThreadsFFT) {
     const int c = e / L, j = e - c * L;
-    const long long row = row0 + c;
-    if (row < nrows) {
-      const int n1 = (int)(row % N1);
-      const long long r2 = row / N1;
-      const int n0 = (int)(r2 % N0);
-      const long long q = r2 / N0;
-      const long long grow = ((q * p.G[0] + grid_of(n0, p.G[0], p.c[0])) * p.G[1] + grid_of(n1, p.G[1], p.c[1])) * L;
-      grid[grow + j] = buf[j * S::NTP + c];
-    }
+    if (c < nr) grid[rdst[c] + j] = buf[j * S::NTP + c];
   }
 }
 
