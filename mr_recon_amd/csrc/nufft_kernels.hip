@@ -627,7 +627,10 @@ namespace mra {
 // blockIdx.y = batch group; problems of one launch share the tile count. A thread owns
 // NQ points of the tile for all RB batch entries, so each covering item gives it
 // NQ * RB independent loads.
-constexpr int kReduceBatch = 4;
+#ifndef MRA_REDUCE_BATCH
+#define MRA_REDUCE_BATCH 4
+#endif
+constexpr int kReduceBatch = MRA_REDUCE_BATCH;
 
 template <int D, int P, int T>
 __global__ __launch_bounds__(kThreads) void grid_reduce_kernel(float2* __restrict__ grid,
