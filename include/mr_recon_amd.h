@@ -73,6 +73,11 @@ int mra_grid_partial(int ndim, int P, const void* ksp, const float* dcf, const f
                      void* scratch, void* stream);
 int mra_grid_reduce(int ndim, int P, void* grid, const void* scratch, const void* geoms, int prob0, int nprob,
                     const int* tile_items, int ntiles, int nbatch, int accumulate, void* stream);
+/* Marks a host Geom record (from mra_geom_fill) so that the gridding kernels index dcf by
+ * sorted (plan) position, dcf_sorted[s] = dcf[perm[s]], instead of through perm. No
+ * reference counterpart (the reference multiplies ksp by dcf before gridding,
+ * linops.py:1208). */
+int mra_geom_set_dcf_plan_order(void* geom_host, int on);
 /* Selects the phase-1 kernel behind mra_grid_partial: 1 (default) = 3D cells of samples
  * summed on the matrix cores (P <= 7), 0 = per-lane LDS read-modify-write. Same scratch
  * output; returns the previous selection. No reference counterpart (one kernel per

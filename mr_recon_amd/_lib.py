@@ -34,6 +34,7 @@ SIGNATURES = {
     "mra_last_error": (C.c_char_p, []),
     "mra_geom_size": (_i, []),
     "mra_set_grid_kernel": (_i, [_i]),
+    "mra_geom_set_dcf_plan_order": (_i, [_vp, _i]),
     "mra_default_tile": (_i, [_i]),
     "mra_grad_samples_per_block": (_i, []),
     "mra_subgrid_points": (_ll, [_i, _i]),

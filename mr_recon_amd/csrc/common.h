@@ -38,7 +38,7 @@ struct Geom {
   int item_base;          // index of this problem's first work item
   long long scratch_off;  // float2 offset of this problem's gridding partials
   int tile_items_off;     // offset of this problem's per-tile item table
-  int pad_;
+  int dcf_plan_order;     // 1: the gridding dcf is indexed by sorted (plan) position, 0: by sample
 };
 
 // Work item: one tile's slice of sorted samples.

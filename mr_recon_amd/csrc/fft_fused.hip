@@ -193,6 +193,12 @@ __global__ __launch_bounds__(kThreadsFFT) void crop_row_kernel(const float2* __r
   const long long gsz = (long long)p.G[0] * p.G[1] * L;
   const long long goff = ((long long)grid_of(n0, p.G[0], p.c[0]) * p.G[1] + grid_of(n1, p.G[1], p.c[1])) * L;
   const long long pix0 = ((long long)n0 * N1 + n1) * N2;
+  // Short rows (N2 <= 128) use all threads: ng groups of pw threads, group h reduces the
+  // coils c = h, h+ng, ... of each chunk for one pixel per thread; the group sums meet in
+  // LDS at the end. Longer rows: one group, up to kCropMaxPix pixels per thread.
+  const int ng = N2 <= kThreadsFFT / 4 ? 4 : (N2 <= kThreadsFFT / 2 ? 2 : 1);
+  const int pw = kThreadsFFT / ng;
+  const int grp = tid / pw, tin = tid - grp * pw;
   float2 acc[kCropMaxPix];
 #pragma unroll
   for (int k = 0; k < kCropMaxPix; ++k) acc[k] = make_float2(0.f, 0.f);
@@ -206,10 +212,10 @@ __global__ __launch_bounds__(kThreadsFFT) void crop_row_kernel(const float2* __r
     fft_lds<L, S>(buf, tw);
 #pragma unroll
     for (int k = 0; k < kCropMaxPix; ++k) {
-      const int n2 = tid + k * kThreadsFFT;
+      const int n2 = tin + k * pw;
       if (n2 < N2) {
         const int g2 = grid_of(n2, L, p.c[2]);
-        for (int c = 0; c < nr; ++c) {
+        for (int c = grp; c < nr; c += ng) {
           float2 v = conjf2(buf[g2 * S::NTP + c]);
           if (smaps) {
             const float2 s = smaps[(qc + c - (coil_off ? 0 : bx * ns)) * s_stride + pix0 + n2];
@@ -222,12 +228,22 @@ __global__ __launch_bounds__(kThreadsFFT) void crop_row_kernel(const float2* __r
     }
     __syncthreads();
   }
+  if (ng > 1) {  // one pixel per thread: fold the groups' sums into group 0
+    if (grp > 0 && tin < N2) buf[grp * pw + tin] = acc[0];
+    __syncthreads();
+    if (grp == 0 && tin < N2)
+      for (int h = 1; h < ng; ++h) {
+        const float2 o = buf[h * pw + tin];
+        acc[0].x += o.x;
+        acc[0].y += o.y;
+      }
+  }
   const float a01 = p.scale * apod_at(p, 0, n0) * apod_at(p, 1, n1);
   float2* orow = out + bx * x_stride + pix0;
 #pragma unroll
   for (int k = 0; k < kCropMaxPix; ++k) {
-    const int n2 = tid + k * kThreadsFFT;
-    if (n2 < N2) {
+    const int n2 = tin + k * pw;
+    if (grp == 0 && n2 < N2) {
       const float a = a01 * apod_at(p, 2, n2);
       float2 v = make_float2(acc[k].x * a, acc[k].y * a);
       if (accumulate) {

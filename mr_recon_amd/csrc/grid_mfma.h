@@ -239,7 +239,7 @@ __global__ __launch_bounds__(64 * MfmaGrid<P>::NW) void grid_mfma_kernel(
     const float* wp = wsort + gs * (3 * P);
 #pragma unroll
     for (int e = 0; e < 3 * P; ++e) rw[e] = ok ? wp[e] : 0.0f;
-    const float sc = (ok && dcf) ? dcf[g.samp_off + o] * g.out_scale : g.out_scale;
+    const float sc = (ok && dcf) ? dcf[g.dcf_plan_order ? gs : g.samp_off + o] * g.out_scale : g.out_scale;
     const float2* kc = ksp + g.ksp_off + (long long)cw0 * g.ksp_stride + o;
 #pragma unroll
     for (int c = 0; c < M::NCW; ++c) {

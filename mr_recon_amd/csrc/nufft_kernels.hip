@@ -322,7 +322,7 @@ __device__ __forceinline__ void stage_load(StageRegs<D, P, RowLanes<D, P, T>::NC
   const float* w = wsort + (ok ? gs : 0) * (D * P);
 #pragma unroll
   for (int e = 0; e < D * P; ++e) r.w[e] = ok ? w[e] : 0.0f;
-  const float sc = (ok && dcf) ? dcf[g.samp_off + o] * g.out_scale : g.out_scale;
+  const float sc = (ok && dcf) ? dcf[g.dcf_plan_order ? gs : g.samp_off + o] * g.out_scale : g.out_scale;
   const float2* kc = ksp + g.ksp_off + (long long)c0 * g.ksp_stride + (ok ? o : 0);
 #pragma unroll
   for (int cs = 0; cs < RL::NC; ++cs) {
@@ -1143,6 +1143,15 @@ int mra_geom_fill(void* dst, long long grid_off, long long grid_stride, long lon
   g.scratch_off = scratch_off;
   g.tile_items_off = tile_items_off;
   *(Geom*)dst = g;
+  MRA_TRY_END
+}
+
+// Mark a host Geom record's dcf as given in sorted (plan) order: the gridding kernels then
+// read it contiguously instead of through perm.
+int mra_geom_set_dcf_plan_order(void* geom_host, int on) {
+  MRA_TRY_BEGIN
+  if (!geom_host) MRA_FAIL("geom: null record");
+  ((Geom*)geom_host)->dcf_plan_order = on ? 1 : 0;
   MRA_TRY_END
 }
 

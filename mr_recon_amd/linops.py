@@ -130,6 +130,8 @@ class sense_linop(linop):
             if use_toeplitz:
                 self.padder = PadLast(self.toep_kerns.shape[-d:], im_size)
                 self.toep_unshifted = toeplitz_unshifted(self.toep_kerns)
+            # dcf in plan order: gridding reads it contiguously instead of through perm
+            self._dcf_plan = self.dcf.reshape(-1)[self.plan.perm.long()].contiguous()
             per_coil = 8 * max(math.prod(self.os_shape), math.prod(self.toep_kerns.shape[-d:]) if use_toeplitz else 0)
             self.coil_chunk = max(1, min(ncoils, MAX_GRID_BYTES // per_coil))
 
@@ -175,7 +177,7 @@ class sense_linop(linop):
         for i, (c0, c1) in enumerate(self._coil_chunks()):
             g = torch.empty((c1 - c0, *self.os_shape), dtype=complex_dtype, device=self.torch_dev)
             sc = self._mark("grid_partial", ops.gridding_partial, y[c0:c1], self.plan, c1 - c0,
-                            dcf=self.dcf.reshape(-1))
+                            dcf=self._dcf_plan, dcf_plan_order=True)
             self._mark("grid_reduce", ops.gridding_reduce, sc, self.plan, g)
             del sc
             self._mark("ifft_crop", ops.ifft_crop, g, self.im_size, self.os_shape, 1, smaps=self.mps[c0:c1],

@@ -69,9 +69,11 @@ def _grid_batch(plan: SamplePlan, out: torch.Tensor) -> int:
 
 
 def gridding_partial(ksp: torch.Tensor, plan: SamplePlan, nbatch: int, dcf: torch.Tensor | None = None,
-                     out_scale: float = 1.0, scratch: torch.Tensor | None = None) -> torch.Tensor:
+                     out_scale: float = 1.0, scratch: torch.Tensor | None = None,
+                     dcf_plan_order: bool = False) -> torch.Tensor:
     """Gridding phase 1: every work item's samples summed into its subgrids in
-    ``scratch`` (returned; allocated when not given)."""
+    ``scratch`` (returned; allocated when not given). ``dcf_plan_order``: ``dcf`` is
+    given in sorted (plan) order, ``dcf[plan.perm]``, and is read contiguously."""
     _lib.require_device(ksp, dcf)
     ksp = _c64(ksp, "ksp")
     B = int(nbatch)
@@ -81,7 +83,7 @@ def gridding_partial(ksp: torch.Tensor, plan: SamplePlan, nbatch: int, dcf: torc
         if dcf.dtype != torch.float32 or dcf.numel() != plan.npts:
             raise ValueError("dcf must be float32 with one entry per sample")
         dcf = dcf.contiguous()
-    g = plan.geom(B, plan.npts, out_scale)
+    g = plan.geom(B, plan.npts, out_scale, dcf_plan_order=dcf is not None and dcf_plan_order)
     need = gridding_scratch_elems(plan, B)
     if scratch is None:
         scratch = torch.empty(need, dtype=torch.complex64, device=ksp.device)

@@ -46,6 +46,7 @@ class GeomKey:
     ksp_stride: int
     out_scale: float
     grid_stride: int
+    dcf_plan_order: bool = False
 
 
 class SamplePlan:
@@ -143,17 +144,19 @@ class SamplePlan:
         return max(1, min(nbatch, self.max_coil_chunk))
 
     def geom(self, nbatch: int, ksp_stride: int | None = None, out_scale: float = 1.0,
-             grid_stride: int | None = None) -> torch.Tensor:
-        """Device Geom table (one entry) for a call on ``nbatch`` grids."""
+             grid_stride: int | None = None, dcf_plan_order: bool = False) -> torch.Tensor:
+        """Device Geom table (one entry) for a call on ``nbatch`` grids; ``dcf_plan_order``:
+        the gridding dcf comes in sorted (plan) order (``dcf[perm]``)."""
         gstride = int(grid_stride if grid_stride is not None else math.prod(self.grid_size))
         kstride = int(ksp_stride if ksp_stride is not None else self.npts)
-        key = GeomKey(int(nbatch), kstride, float(out_scale), gstride)
+        key = GeomKey(int(nbatch), kstride, float(out_scale), gstride, bool(dcf_plan_order))
         g = self._geoms.get(key)
         if g is None:
             g = make_geoms([dict(grid_off=0, grid_stride=gstride, ksp_off=0, ksp_stride=kstride, samp_off=0,
                                  nbatch=int(nbatch), npts=self.npts, grid_size=self.grid_size, tile=self.tile,
                                  wrap=self.wrap, beta=self.beta, width=self.width, out_scale=out_scale,
-                                 item_base=0, scratch_off=0, tile_items_off=0)],
+                                 item_base=0, scratch_off=0, tile_items_off=0,
+                                 dcf_plan_order=bool(dcf_plan_order))],
                            self.device)
             self._geoms[key] = g
         return g
@@ -170,5 +173,7 @@ def make_geoms(entries: list[dict], device) -> torch.Tensor:
                               e["ksp_stride"], e["samp_off"], e["nbatch"], e["npts"], D, _lib.ints(e["grid_size"]),
                               e["tile"], _lib.ints(e["wrap"]), e["beta"], e["width"], e["out_scale"],
                               e.get("item_base", 0), e.get("scratch_off", 0), e.get("tile_items_off", 0)))
+        if e.get("dcf_plan_order"):
+            check(L.mra_geom_set_dcf_plan_order(C.addressof(buf) + i * gsz, 1))
     host = torch.frombuffer(bytearray(bytes(buf)), dtype=torch.uint8)
     return host.to(device)

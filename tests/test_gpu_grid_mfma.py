@@ -92,3 +92,27 @@ def test_mfma_grid_kooshball_many_coils(dev):
     y = torch.randn((32, k.shape[0]), dtype=torch.complex64, generator=torch.Generator().manual_seed(5))
     mfma, lane = _grid_both(dev, k.float(), y, None, G, 4.0, 9.1)
     assert rel_l2(mfma, lane) < TOL
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_dcf_in_plan_order(dev, D):
+    """dcf handed over in sorted (plan) order (dcf[perm], read contiguously) grids the same
+    as dcf in sample order, for both phase-1 kernels."""
+    from mr_recon_amd import ops
+    from mr_recon_amd.plan import SamplePlan
+
+    G = 32
+    k = torch.from_numpy(edge_coords((G,) * D, 4000, seed=D, width=4.0))
+    y = torch.randn((5, k.shape[0]), dtype=torch.complex64, generator=torch.Generator().manual_seed(D)).to(dev)
+    dcf = (torch.rand(k.shape[0], generator=torch.Generator().manual_seed(9)) + 0.5).to(dev)
+    plan = SamplePlan(k.to(dev), (G,) * D, 4.0, 9.0)
+    ref = interp_ref.grid(y.cpu() * dcf.cpu(), k, (G,) * D, 4.0, beta=9.0)
+    for mode in (1, 0):
+        with grid_kernel(mode):
+            a = torch.empty((5, *(G,) * D), dtype=torch.complex64, device=dev)
+            b = torch.empty_like(a)
+            ops.gridding_reduce(ops.gridding_partial(y, plan, 5, dcf=dcf), plan, a)
+            ops.gridding_reduce(ops.gridding_partial(y, plan, 5, dcf=dcf[plan.perm.long()].contiguous(),
+                                                     dcf_plan_order=True), plan, b)
+        assert rel_l2(a, ref) < TOL
+        assert torch.equal(a, b)
