@@ -632,6 +632,65 @@ namespace mra {
 #endif
 constexpr int kReduceBatch = MRA_REDUCE_BATCH;
 
+// grid_reduce for 3D with every G >= SUB and even G[2]: thread t owns the points
+// (z, y, x), (z, y, x + 1) of the output tile, x even, for kReduceBatch batch entries;
+// each covering item's pair is one 16-byte load per entry.
+template <int P, int T>
+__device__ __forceinline__ void reduce_pairs_3d(float2* __restrict__ grid, const float2* __restrict__ sbase,
+                                                const Geom& g, const int (&tc)[3], const int (&cand)[3][3],
+                                                const int (&ncand)[3], const int* __restrict__ titems, int b0,
+                                                int nb, int accumulate) {
+  using TS = TileShape<3, P, T>;
+  constexpr int RB = kReduceBatch;
+  const int t = threadIdx.x;
+  const int xp = t % (T / 2), rest = t / (T / 2);
+  const int gc[3] = {tc[0] * T + rest / T, tc[1] * T + rest % T, tc[2] * T + 2 * xp};
+  const bool inside = gc[0] < g.G[0] && gc[1] < g.G[1] && gc[2] < g.G[2];
+  float4 acc[RB];
+#pragma unroll
+  for (int b = 0; b < RB; ++b) acc[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int k0 = 0; k0 < ncand[0]; ++k0)
+    for (int k1 = 0; k1 < ncand[1]; ++k1)
+      for (int k2 = 0; k2 < ncand[2]; ++k2) {
+        const int c0 = cand[0][k0], c1 = cand[1][k1], c2 = cand[2][k2];
+        const int l0 = pmod(gc[0] - c0 * T, g.G[0]), l1 = pmod(gc[1] - c1 * T, g.G[1]);
+        const int l2 = pmod(gc[2] - c2 * T, g.G[2]);  // even, so l2 + 1 < SUB when l2 < SUB
+        const bool ok = inside && l0 < TS::SUB && l1 < TS::SUB && l2 < TS::SUB;
+        const int e = (l0 * TS::SUB + l1) * TS::SUB + l2;
+        const int stile = (c0 * g.ntile[1] + c1) * g.ntile[2] + c2;
+        const int i1 = titems[stile + 1];
+        for (int it = titems[stile]; it < i1; ++it) {
+          const float2* slot = sbase + ((long long)(it - g.item_base) * g.nbatch + b0) * TS::SUBN + e;
+          if (ok) {
+#pragma unroll
+            for (int b = 0; b < RB; ++b)
+              if (b < nb) {
+                const float4 v = *(const float4*)(slot + (long long)b * TS::SUBN);
+                acc[b].x += v.x;
+                acc[b].y += v.y;
+                acc[b].z += v.z;
+                acc[b].w += v.w;
+              }
+          }
+        }
+      }
+  if (!inside) return;
+  const long long lin = ((long long)gc[0] * g.G[1] + gc[1]) * g.G[2] + gc[2];
+#pragma unroll
+  for (int b = 0; b < RB; ++b) {
+    if (b >= nb) break;
+    float4* o = (float4*)(grid + g.grid_off + (long long)(b0 + b) * g.grid_stride + lin);
+    if (accumulate) {
+      const float4 v = *o;
+      acc[b].x += v.x;
+      acc[b].y += v.y;
+      acc[b].z += v.z;
+      acc[b].w += v.w;
+    }
+    *o = acc[b];
+  }
+}
+
 template <int D, int P, int T>
 __global__ __launch_bounds__(kThreads) void grid_reduce_kernel(float2* __restrict__ grid,
                                                                const float2* __restrict__ scratch,
@@ -688,6 +747,16 @@ __global__ __launch_bounds__(kThreads) void grid_reduce_kernel(float2* __restric
   const int* titems = tile_items + g.tile_items_off;
   const float2* sbase = scratch + g.scratch_off;
   const bool simple = g.G[0] >= TS::SUB && (D < 2 || g.G[1] >= TS::SUB) && (D < 3 || g.G[D - 1] >= TS::SUB);
+  if constexpr (D == 3 && T % 2 == 0 && TS::SUB % 2 == 0 && TS::TN == 2 * kThreads) {
+    // two adjacent points along the last dim per thread, 16-byte loads and stores; needs
+    // even offsets so that a pair never straddles the wrap and stays 16-byte aligned
+    const bool pairs = simple && g.G[2] % 2 == 0 && g.scratch_off % 2 == 0 && g.grid_off % 2 == 0 &&
+                       g.grid_stride % 2 == 0;
+    if (pairs) {
+      reduce_pairs_3d<P, T>(grid, sbase, g, tc, cand, ncand, titems, b0, nb, accumulate);
+      return;
+    }
+  }
   // iterate over candidate tile combinations (row-major over dims; uniform per block)
   int k[D];
 #pragma unroll
