@@ -751,7 +751,7 @@ __global__ __launch_bounds__(kThreads) void grid_reduce_kernel(float2* __restric
     // two adjacent points along the last dim per thread, 16-byte loads and stores; needs
     // even offsets so that a pair never straddles the wrap and stays 16-byte aligned
     const bool pairs = simple && g.G[2] % 2 == 0 && g.scratch_off % 2 == 0 && g.grid_off % 2 == 0 &&
-                       g.grid_stride % 2 == 0;
+                       g.grid_stride % 2 == 0 && (uintptr_t)grid % 16 == 0 && (uintptr_t)scratch % 16 == 0;
     if (pairs) {
       reduce_pairs_3d<P, T>(grid, sbase, g, tc, cand, ncand, titems, b0, nb, accumulate);
       return;
