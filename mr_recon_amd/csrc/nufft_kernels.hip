@@ -49,7 +49,10 @@ template <> struct TileCfg<1> { static constexpr int T = 256; };
 template <> struct TileCfg<2> { static constexpr int T = 16; };
 template <> struct TileCfg<3> { static constexpr int T = 8; };
 
-constexpr int kLdsBudget = 32 * 1024;  // bytes of LDS subgrids per interp block (more, smaller blocks hide latency better)
+#ifndef MRA_INTERP_LDS_KB
+#define MRA_INTERP_LDS_KB 32
+#endif
+constexpr int kLdsBudget = MRA_INTERP_LDS_KB * 1024;  // bytes of LDS subgrids per interp block (more, smaller blocks hide latency better)
 
 // Offset in the subgrid of patch point q of the trailing D-1 dims.
 template <int D, int P, int T>
@@ -211,6 +214,98 @@ __global__ __launch_bounds__(kThreads) void interp_kernel(const float2* __restri
         ay += wx[i] * by;
       }
       odst[(long long)cl * g.ksp_stride + o] = make_float2(ax, ay);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------------------------
+// Interpolation with the coils of a chunk in pairs: the LDS subgrids are stored coil-pair
+// interleaved (one float4 = two coils' values of a grid point), so every patch point is
+// one 128-bit LDS read serving both coils (gfx950 moves ~2.7x the bytes per clock with
+// 128-bit reads than with 64-bit ones). Same work decomposition as interp_kernel; an odd
+// coil count pads its last pair with zeros.
+template <int D, int P, int T>
+__global__ __launch_bounds__(kThreads) void interp_pair_kernel(const float2* __restrict__ grid,
+                                                               float2* __restrict__ out,
+                                                               const float* __restrict__ wsort,
+                                                               const int* __restrict__ lbase,
+                                                               const int* __restrict__ perm,
+                                                               const Geom* __restrict__ geoms,
+                                                               const Item* __restrict__ items, int nitems,
+                                                               int ccoils, int nchunks) {
+  using TS = TileShape<D, P, T>;
+  extern __shared__ float4 lds4[];
+  int iidx, cc;
+  item_of_block(blockIdx.x, nchunks, iidx, cc);
+  if (iidx >= nitems) return;
+  const Item it = items[iidx];
+  if (it.w <= 0) return;
+  const Geom g = geoms[it.x];
+  const int c0 = cc * ccoils;
+  if (c0 >= g.nbatch) return;
+  const int nc = min(ccoils, g.nbatch - c0);
+  const int npair = (nc + 1) / 2;
+  const int tid = threadIdx.x;
+  float2* odst = out + g.ksp_off + (long long)c0 * g.ksp_stride;
+
+  if (it.y < 0) {  // samples with non-finite coordinates: defined as 0
+    for (int s = it.z + tid; s < it.z + it.w; s += kThreads) {
+      const int o = perm[g.samp_off + s];
+      for (int cl = 0; cl < nc; ++cl) odst[(long long)cl * g.ksp_stride + o] = make_float2(0.f, 0.f);
+    }
+    return;
+  }
+
+  int tc[D];
+  tile_coords<D, T>(it.y, g, tc);
+  {
+    int goff[TS::NQ];
+    subgrid_offsets<D, P, T>(g, tc, goff);
+    const float2* gsrc = grid + g.grid_off + (long long)c0 * g.grid_stride;
+    for (int cp = 0; cp < npair; ++cp) {
+      const bool two = 2 * cp + 1 < nc;
+#pragma unroll
+      for (int q = 0; q < TS::NQ; ++q)
+        if (goff[q] >= 0) {
+          const float2 a = gsrc[(long long)(2 * cp) * g.grid_stride + goff[q]];
+          const float2 b = two ? gsrc[(long long)(2 * cp + 1) * g.grid_stride + goff[q]] : make_float2(0.f, 0.f);
+          lds4[cp * TS::SUBN + tid + q * kThreads] = make_float4(a.x, a.y, b.x, b.y);
+        }
+    }
+  }
+  __syncthreads();
+
+  for (int s = it.z + tid; s < it.z + it.w; s += kThreads) {
+    const long long gs = g.samp_off + s;
+    float wx[P], wr[TS::PR];
+    load_weights<D, P>(wsort, gs, g.out_scale, wx, wr);
+    const int base = lbase[gs];
+    const int o = perm[gs];
+    for (int cp = 0; cp < npair; ++cp) {
+      const float4* sb = lds4 + cp * TS::SUBN + base;
+      float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
+#ifdef MRA_INTERP_PAIR_UNROLL
+#pragma unroll
+#else
+#pragma unroll 1
+#endif
+      for (int i = 0; i < P; ++i) {
+        float bx = 0.f, by = 0.f, bz = 0.f, bw = 0.f;
+#pragma unroll
+        for (int q = 0; q < TS::PR; ++q) {
+          const float4 v = sb[i * TS::SUBD1 + rest_off<D, P, T>(q)];
+          bx += wr[q] * v.x;
+          by += wr[q] * v.y;
+          bz += wr[q] * v.z;
+          bw += wr[q] * v.w;
+        }
+        ax += wx[i] * bx;
+        ay += wx[i] * by;
+        az += wx[i] * bz;
+        aw += wx[i] * bw;
+      }
+      odst[(long long)(2 * cp) * g.ksp_stride + o] = make_float2(ax, ay);
+      if (2 * cp + 1 < nc) odst[(long long)(2 * cp + 1) * g.ksp_stride + o] = make_float2(az, aw);
     }
   }
 }
@@ -1046,6 +1141,11 @@ struct LaunchArgs {
 
 // Phase-1 gridding kernel: 1 = matrix cores (3D, P <= 7), 0 = per-lane LDS read-modify-write.
 static int g_grid_kernel = 1;
+// Interpolation reads coil pairs as 128-bit LDS values when the coil chunk is even.
+#ifndef MRA_INTERP_PAIRS
+#define MRA_INTERP_PAIRS 1
+#endif
+static int g_interp_pairs = MRA_INTERP_PAIRS;
 
 template <int D, int P>
 static int launch_op(const LaunchArgs& a) {
@@ -1059,6 +1159,13 @@ static int launch_op(const LaunchArgs& a) {
       break;
     case Op::Interp: {
       if (a.nitems <= 0) break;
+      if (g_interp_pairs && a.ccoils % 2 == 0) {
+        const size_t sh = (size_t)a.ccoils * TS::SUBN * sizeof(float2);
+        interp_pair_kernel<D, P, T><<<item_grid(a.nitems, a.nchunks), kThreads, sh, a.stream>>>(
+            (const float2*)a.src, (float2*)a.dst, a.wsort, a.lbase, a.perm, a.geoms, a.items, a.nitems, a.ccoils,
+            a.nchunks);
+        break;
+      }
       const size_t sh = (size_t)a.ccoils * TS::SUBN * sizeof(float2);
       interp_kernel<D, P, T><<<item_grid(a.nitems, a.nchunks), kThreads, sh, a.stream>>>(
           (const float2*)a.src, (float2*)a.dst, a.wsort, a.lbase, a.perm, a.geoms, a.items, a.nitems, a.ccoils,
