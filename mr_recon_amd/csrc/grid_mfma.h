@@ -14,8 +14,8 @@
 // samples. A wave owns NCW coils and keeps R in registers while it walks a cell. The
 // ring values of the cell's region are read before the cell's last group of products is
 // issued and written back (old + R) right after it, so the read latency hides behind the
-// matrix work; each lane's 4 result values (re and im of 2 coils, or of one coil at two
-// p2: 16 contiguous bytes) take one 128-bit read and one write per 16x16 block. Compared
+// matrix work; each lane's 4 result values (re and im of 2 coils at one p2: 16 contiguous
+// bytes) take one 128-bit read and one write per 16x16 block. Compared
 // with one LDS read-modify-write of P^3 points per sample and coil, LDS traffic drops by
 // the samples per cell (18 on average for a 3D kooshball) and the multiply-adds move to
 // the matrix cores.
