@@ -727,6 +727,63 @@ namespace mra {
 #endif
 constexpr int kReduceBatch = MRA_REDUCE_BATCH;
 
+// grid_reduce for 2D with every G >= SUB and even G[1]: the T x T tile has T*T/2 point
+// pairs (y, x), (y, x + 1), x even; thread t owns pair t % (T*T/2) for batch entries
+// 2h, 2h+1 of the block's four, h = t / (T*T/2), one 16-byte load per covering item each.
+template <int P, int T>
+__device__ __forceinline__ void reduce_pairs_2d(float2* __restrict__ grid, const float2* __restrict__ sbase,
+                                                const Geom& g, const int (&tc)[2], const int (&cand)[2][3],
+                                                const int (&ncand)[2], const int* __restrict__ titems, int b0,
+                                                int nb, int accumulate) {
+  using TS = TileShape<2, P, T>;
+  constexpr int NPAIR = T * T / 2;
+  const int t = threadIdx.x;
+  const int pr = t % NPAIR, h = t / NPAIR;
+  const int gc[2] = {tc[0] * T + pr / (T / 2), tc[1] * T + 2 * (pr % (T / 2))};
+  const bool inside = gc[0] < g.G[0] && gc[1] < g.G[1];
+  float4 acc[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+  for (int k0 = 0; k0 < ncand[0]; ++k0)
+    for (int k1 = 0; k1 < ncand[1]; ++k1) {
+      const int c0 = cand[0][k0], c1 = cand[1][k1];
+      const int l0 = pmod(gc[0] - c0 * T, g.G[0]);
+      const int l1 = pmod(gc[1] - c1 * T, g.G[1]);  // even, so l1 + 1 < SUB when l1 < SUB
+      const bool ok = inside && l0 < TS::SUB && l1 < TS::SUB;
+      const int e = l0 * TS::SUB + l1;
+      const int stile = c0 * g.ntile[1] + c1;
+      const int i1 = titems[stile + 1];
+      for (int it = titems[stile]; it < i1; ++it) {
+        const float2* slot = sbase + ((long long)(it - g.item_base) * g.nbatch + b0) * TS::SUBN + e;
+        if (ok) {
+#pragma unroll
+          for (int u = 0; u < 2; ++u)
+            if (2 * h + u < nb) {
+              const float4 v = *(const float4*)(slot + (long long)(2 * h + u) * TS::SUBN);
+              acc[u].x += v.x;
+              acc[u].y += v.y;
+              acc[u].z += v.z;
+              acc[u].w += v.w;
+            }
+        }
+      }
+    }
+  if (!inside) return;
+  const long long lin = (long long)gc[0] * g.G[1] + gc[1];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int b = 2 * h + u;
+    if (b >= nb) break;
+    float4* o = (float4*)(grid + g.grid_off + (long long)(b0 + b) * g.grid_stride + lin);
+    if (accumulate) {
+      const float4 v = *o;
+      acc[u].x += v.x;
+      acc[u].y += v.y;
+      acc[u].z += v.z;
+      acc[u].w += v.w;
+    }
+    *o = acc[u];
+  }
+}
+
 // grid_reduce for 3D with every G >= SUB and even G[2]: thread t owns the points
 // (z, y, x), (z, y, x + 1) of the output tile, x even, for kReduceBatch batch entries;
 // each covering item's pair is one 16-byte load per entry.
@@ -842,6 +899,14 @@ __global__ __launch_bounds__(kThreads) void grid_reduce_kernel(float2* __restric
   const int* titems = tile_items + g.tile_items_off;
   const float2* sbase = scratch + g.scratch_off;
   const bool simple = g.G[0] >= TS::SUB && (D < 2 || g.G[1] >= TS::SUB) && (D < 3 || g.G[D - 1] >= TS::SUB);
+  if constexpr (D == 2 && T % 2 == 0 && TS::SUB % 2 == 0 && TS::TN == kThreads && kReduceBatch == 4) {
+    const bool pairs = simple && g.G[1] % 2 == 0 && g.scratch_off % 2 == 0 && g.grid_off % 2 == 0 &&
+                       g.grid_stride % 2 == 0 && (uintptr_t)grid % 16 == 0 && (uintptr_t)scratch % 16 == 0;
+    if (pairs) {
+      reduce_pairs_2d<P, T>(grid, sbase, g, tc, cand, ncand, titems, b0, nb, accumulate);
+      return;
+    }
+  }
   if constexpr (D == 3 && T % 2 == 0 && TS::SUB % 2 == 0 && TS::TN == 2 * kThreads) {
     // two adjacent points along the last dim per thread, 16-byte loads and stores; needs
     // even offsets so that a pair never straddles the wrap and stays 16-byte aligned
