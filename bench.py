@@ -5,7 +5,9 @@ captured graph against a loop of plain single-problem calls; ``grad`` -- loss an
 gradients (image and trajectory) through a 2D SENSE NUFFT plus an unrolled CG-SENSE
 against the plain per-coil differentiable path; ``cg2d`` -- 2D spiral CG-SENSE, 30
 iterations, Toeplitz normal operator; ``cg3d`` -- the 3D kooshball above, 15 CG-SENSE
-iterations (Toeplitz and NUFFT normal operators). Each prints its own JSON line.
+iterations (Toeplitz and NUFFT normal operators); ``dcf`` -- Pipe-Menon density
+compensation of the kooshball trajectory below, 30 iterations. Each prints its own
+JSON line.
 
 Workload (the headline configuration): 128^3 complex64 image, 32 smooth coil
 maps, 20,000 Fibonacci-sphere spokes x 256 readout points (5.12 M samples per
@@ -368,13 +370,62 @@ def run_cg(args, three_d: bool):
     print(json.dumps(rec), flush=True)
 
 
+def run_dcf(args):
+    """Pipe-Menon density compensation (30 iterations of w /= |G^H G w|, KB width 4,
+    beta 8, on the 128^3 image grid) of the headline 3D kooshball trajectory."""
+    from mr_recon_amd import synth
+    from mr_recon_amd.algs import density_compensation
+
+    dev = torch.device("cuda:0")
+    iters = 30
+    trj = synth.kooshball_3d(NSPOKES, NREAD, N).to(dev)
+    M = trj.shape[0] * trj.shape[1]
+    for _ in range(args.warmup):
+        density_compensation(trj, (N, N, N), iters)
+    t, d = _timed(lambda: density_compensation(trj, (N, N, N), iters), args.steps)
+    log(f"dcf: {t * 1e3:.2f} ms per call ({t * 1e3 / iters:.3f} ms per iteration), max {float(d.max())}")
+    # per iteration: gridding reads w (4M) and the plan's sorted coordinates (12M) and
+    # writes the grid (8N^3); interpolation reads the grid and coordinates and writes
+    # G^H G w (8M); the update reads it and w and writes w (12M)
+    it_bytes = 4 * M + 12 * M + 8 * N ** 3 + 8 * N ** 3 + 12 * M + 8 * M + 12 * M
+    achieved = it_bytes / (t / iters) / 1e9
+    rec = {
+        "metric": f"Pipe-Menon DCFs/s (3D {N}^3 grid, {NSPOKES} kooshball spokes x {NREAD}, {iters} iterations)",
+        "value": 1.0 / t, "unit": "dcf/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded kooshball trajectory)",
+        "config": {"workload": "density_compensation (Pipe-Menon)", "image": [N, N, N], "samples": M,
+                   "iterations": iters, "width": 4, "beta": 8},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "roofline_detail": {"kernel": "one iteration (gridding + interpolation + update)",
+                            "avg_ms": t * 1e3 / iters, "algorithmic_bytes_per_iteration": it_bytes,
+                            "bytes_formula": "48*M + 16*N^3"},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import dcf_ref
+
+        nsp = 1200
+        sub = synth.kooshball_3d(nsp, NREAD, N)
+        t0 = time.perf_counter()
+        dcf_ref.pipe_menon(sub, (N, N, N), max_iter=1, dtype=torch.float32)
+        tc = time.perf_counter() - t0
+        per_call = tc * (M / (nsp * NREAD)) * iters
+        rec["cpu_baseline"] = {"value": 1.0 / per_call, "unit": "dcf/s", "cores": torch.get_num_threads(),
+                               "kind": "port",
+                               "sample": f"one Pipe-Menon iteration of the oracle (fp32 torch on CPU) on {nsp} "
+                                         f"spokes x {NREAD}, scaled by samples and x{iters} iterations "
+                                         f"({tc:.2f} s measured)"}
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="nufft3d", choices=["nufft3d", "batched", "grad", "cg2d", "cg3d"])
+    ap.add_argument("--workload", default="nufft3d", choices=["nufft3d", "batched", "grad", "cg2d", "cg3d", "dcf"])
     ap.add_argument("--samples-per-item", default="auto",
                     help="batched workload: work-item size (an int, or 'auto' for the autotuner)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -393,6 +444,8 @@ def main():
         return run_grad(args)
     if args.workload in ("cg2d", "cg3d"):
         return run_cg(args, args.workload == "cg3d")
+    if args.workload == "dcf":
+        return run_dcf(args)
 
     from mr_recon_amd.fourier import hip_nufft
     from mr_recon_amd.linops import sense_linop

@@ -148,6 +148,14 @@ int mra_toeplitz_fused(int ndim, const int* N, const int* G, const int* center, 
                        long long ns, long long s_stride, const void* kern, void* work, int accumulate, void* out,
                        const void* const* tw, void* stream);
 
+/* ---- density compensation ---------------------------------------------------------------
+ * Elementwise steps of the Pipe-Menon iteration behind density_compensation (algs.py:20-106,
+ * which calls sigpy.mri.pipe_menon_dcf): w /= |G^H G w| after the gridding and interpolation
+ * calls above, and the final dcf /= max(dcf) (algs.py:105). part: mra_dcf_partials() floats. */
+int mra_dcf_partials(void);
+int mra_dcf_update(const void* ghg, float* w, long long n, void* stream);
+int mra_dcf_normalize(float* w, long long n, float* part, void* stream);
+
 /* ---- conjugate gradient -----------------------------------------------------------------
  * Vector work of conjugate_gradient (algs.py:779-872) with FP64 inner products
  * and device-resident step sizes. */

@@ -1,4 +1,5 @@
-"""Solvers on the NUFFT path: ``conjugate_gradient`` and ``power_method_operator``.
+"""Solvers on the NUFFT path: ``conjugate_gradient``, ``power_method_operator`` and
+``density_compensation``.
 
 Same signatures and iteration as the reference (algs.py:364-399, 779-872):
 x0 = AHb, r = AHb - (AHA x0 + lamda_l2 x0), p = r; per iteration Ap, pAp,
@@ -20,6 +21,7 @@ from typing import Optional, Tuple
 import torch
 import torch.nn as nn
 
+from . import _lib
 from ._lib import check, lib, ptr, stream_ptr
 
 
@@ -127,3 +129,58 @@ def power_method_operator(A: callable, x0: torch.Tensor, num_iter: Optional[int]
 
 
 __all__ = ["conjugate_gradient", "power_method_operator", "CGWorkspace"]
+
+
+# Kaiser-Bessel gridding of the Pipe-Menon iteration: sigpy.mri.pipe_menon_dcf's
+# defaults (width=4, beta=8), which density_compensation (algs.py:61-64) keeps.
+PIPE_MENON_WIDTH = 4.0
+PIPE_MENON_BETA = 8.0
+
+
+def density_compensation(trj: torch.Tensor, im_size: tuple, num_iters: Optional[int] = 30,
+                         method: str = "sigpy") -> torch.Tensor:
+    """Pipe-Menon density compensation factors, shape ``trj.shape[:-1]`` (algs.py:20-106).
+
+    ``trj`` (..., d) float32 device tensor in grid units (-N/2 .. N/2). The reference
+    hands the trajectory to ``sigpy.mri.pipe_menon_dcf(trj, im_size, max_iter=num_iters)``:
+    w = 1, then ``num_iters`` times w /= |G^H G w| with G the Kaiser-Bessel gridding
+    (width 4, beta 8) of the samples onto an ``im_size`` grid -- no oversampling, no
+    shift, grid indices wrapping mod N -- and finally dcf /= max(dcf) (algs.py:105).
+
+    Here G is the HIP gridding (with w as its per-sample weight on unit samples) and
+    G^H the HIP interpolation over one sample plan of the trajectory; the division
+    and the normalisation are HIP kernels. Only ``method='sigpy'`` exists in the
+    reference (its other branches are commented out).
+    """
+    from . import ops
+    from .plan import SamplePlan
+
+    method = method.lower()
+    if method != "sigpy":
+        raise ValueError(f"density_compensation: unknown method {method!r} (only 'sigpy' is implemented)")
+    _lib.require_device(trj)
+    D = int(trj.shape[-1])
+    im_size = tuple(int(n) for n in im_size)
+    if len(im_size) != D:
+        raise ValueError(f"im_size {im_size} does not match trajectory dimension {D}")
+    if num_iters is None or num_iters < 0:
+        raise ValueError("num_iters must be a non-negative integer")
+    coords = trj.reshape(-1, D).to(torch.float32).contiguous()
+    plan = SamplePlan(coords, im_size, PIPE_MENON_WIDTH, PIPE_MENON_BETA)
+    M = plan.npts
+    dev = trj.device
+    st = stream_ptr(dev)
+    L = lib()
+    w = torch.ones(M, dtype=torch.float32, device=dev)
+    if num_iters > 0:
+        unit = torch.ones((1, M), dtype=torch.complex64, device=dev)
+        grid = torch.empty((1, *im_size), dtype=torch.complex64, device=dev)
+        ghg = torch.empty((1, M), dtype=torch.complex64, device=dev)
+        scratch = torch.empty(ops.gridding_scratch_elems(plan, 1), dtype=torch.complex64, device=dev)
+        for _ in range(int(num_iters)):
+            ops.gridding(unit, plan, grid, dcf=w, scratch=scratch)
+            ops.interp(grid, plan, out=ghg)
+            check(L.mra_dcf_update(ptr(ghg), ptr(w), M, st))
+    part = torch.empty(L.mra_dcf_partials(), dtype=torch.float32, device=dev)
+    check(L.mra_dcf_normalize(ptr(w), M, ptr(part), st))
+    return w.reshape(trj.shape[:-1])

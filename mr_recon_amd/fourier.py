@@ -5,6 +5,7 @@ Mirrors the reference ``mr_recon.fourier`` for the hot path:
   ``calc_toep_kernel_helper``   fourier.py:63-105
   ``NUFFT``                     fourier.py:107-352 (base class, ``normal_toeplitz``)
   ``hip_nufft``                 fourier.py:354-582 (``triton_nufft``; alias kept)
+  ``sigpy_nufft``               fourier.py:728-1033 (same kernels, sigpy's apodization)
 
 Transform definition (the reference's): forward
     ksp = interp_KB( FFT( pad_G( apod * img / sqrt(prod N) ) ) ) / w^d
@@ -366,5 +367,91 @@ class hip_nufft(NUFFT):
 # switches by changing the import only.
 triton_nufft = hip_nufft
 
+
+def sigpy_beta(width: float, oversamp: float) -> float:
+    """sigpy_nufft's default beta (fourier.py:740-747): the usual KB formula, or 1.0
+    when its radicand is negative (with the reference's warning unless width == 1)."""
+    if ((width / oversamp) * (oversamp - 0.5)) ** 2 - 0.8 < 0:
+        if width != 1:
+            print("WARNING: beta is set to 1.0")
+        return 1.0
+    return kb_beta(width, oversamp)
+
+
+class sigpy_nufft(hip_nufft):
+    """Drop-in for the reference's ``sigpy_nufft`` (fourier.py:728-1033) on the HIP kernels.
+
+    The reference runs sigpy's Kaiser-Bessel ``interpolate`` / ``gridding`` on the
+    scaled trajectory (fourier.py:901-963): the same transform as ``triton_nufft`` --
+    apodize, / sqrt(prod N), centred pad to ceil(s N), centred FFT (norm=None), KB
+    interpolation / width^d -- so ``forward``, ``adjoint`` and the split methods are
+    ``hip_nufft``'s fused kernels. What differs is kept here: the beta default falls
+    back to 1.0 (``sigpy_beta``), the apodization divides by sinh + 1e-12 and continues
+    to sqrt(-a) / sin(sqrt(-a)) where beta^2 < (pi w x)^2 (fourier.py:817-841), and the
+    Toeplitz kernels come from a ``sigpy_nufft`` of the oversampled size.
+    """
+
+    def __init__(self, im_size: tuple, oversamp: Optional[float] = 1.25, width: Optional[int] = 6,
+                 beta: Optional[float] = None, apodize: Optional[bool] = True,
+                 samples_per_item=None, tune_batch: Optional[int] = None):
+        super().__init__(im_size, oversamp=oversamp, width=width,
+                         beta=sigpy_beta(width, oversamp) if beta is None else beta, apodize=apodize,
+                         samples_per_item=samples_per_item, tune_batch=tune_batch)
+
+    def _apodization_func_kb(self, x: torch.Tensor, beta: float, width: float) -> torch.Tensor:
+        """KB apodization with its continuation past beta = pi w |x| (fourier.py:817-841)."""
+        eps = 1e-12
+        arg = beta ** 2 - (np.pi * width * x) ** 2
+        pos = arg.clamp(min=0).sqrt()
+        pos = pos / (torch.sinh(pos) + eps)
+        neg = (-arg.clamp(max=0)).sqrt()
+        neg = neg / (torch.sin(neg) + eps)
+        return pos + neg
+
+    def _apodization_func_rect(self, x: torch.Tensor) -> torch.Tensor:
+        """Apodization of a rectangular kernel (fourier.py:800-815)."""
+        return 1 / torch.sinc(self.width * x / self.oversamp)
+
+    def _apodization_func(self, x: torch.Tensor) -> torch.Tensor:
+        return self._apodization_func_kb(x, self.beta, self.width)
+
+    def calc_teoplitz_kernels(self, trj: torch.Tensor, weights: Optional[torch.Tensor] = None,
+                              os_factor: Optional[float] = 2.0):
+        """Centred PSF kernels (N, *round(os_factor * im_size)) (fourier.py:1007-1033); the
+        oversampled operator is a new ``sigpy_nufft`` with the default beta."""
+        im_size_os = tuple(round(i * os_factor) for i in self.im_size)
+        nufft_os = sigpy_nufft(im_size=im_size_os, oversamp=self.oversamp, width=self.width,
+                               samples_per_item=self.samples_per_item, tune_batch=1)
+        return calc_toep_kernel_helper(nufft_os.adjoint, trj * os_factor, weights) * (os_factor ** len(self.im_size))
+
+    def optimal_beta(self, nbetas: Optional[int] = 100, num_sigs: Optional[int] = 1000,
+                     torch_dev: Optional[torch.device] = None) -> float:
+        """The beta on torch.linspace(0.5, 20, nbetas) whose 1D NUFFT, least-squares
+        scaled, best matches the exact DFT of ``num_sigs`` random signals at N offsets
+        within +-0.5/oversamp of the grid (fourier.py:749-798). The signals are drawn
+        from torch's CPU generator as in the reference (its default device is the CPU);
+        the transforms run on ``torch_dev`` (default: the current GPU)."""
+        from .utils import gen_grd
+
+        dev = torch.device("cuda", torch.cuda.current_device()) if torch_dev is None else torch.device(torch_dev)
+        N = max(self.im_size)
+        kdevs = torch.linspace(-0.5, 0.5, N) / self.oversamp
+        betas = torch.linspace(0.5, 20, nbetas)
+        rnd_sigs = torch.randn((num_sigs, N), dtype=complex_dtype)
+        rs = gen_grd((N,))[:, 0]
+        gt = torch.exp(-2j * np.pi * (kdevs[:, None] @ rs[None, :]))
+        vals_gt = (rnd_sigs @ gt.T).to(dev)
+        sigs = rnd_sigs.to(dev)
+        trj = kdevs[None, :, None].to(dev)
+        errs = []
+        for beta in betas:
+            nft = sigpy_nufft((self.im_size[0],), self.oversamp, self.width, beta=float(beta),
+                              samples_per_item=self.samples_per_item)
+            vals = nft.forward(sigs[None], trj)[0]
+            scale = (vals.conj() * vals_gt).sum() / (vals.conj() * vals).sum()
+            errs.append((vals * scale - vals_gt).norm())
+        return betas[int(torch.stack(errs).argmin())].item()
+
 __all__ = ["fft", "ifft", "calc_toep_kernel_helper", "NUFFT", "hip_nufft", "triton_nufft", "kb_beta",
+           "sigpy_nufft", "sigpy_beta",
            "toeplitz_unshifted", "batch_iterator"]

@@ -22,4 +22,9 @@ How each part is pinned (tests/test_oracle_*.py):
   sense_ref.py   SENSE forward/adjoint identities vs explicit per-coil loops;
                  CG vs torch.linalg.solve on a small dense SPD system.
   ndft.py        closed form; checked on integer k against numpy's FFT.
+  dcf_ref.py     Pipe-Menon iteration vs the same iteration on a dense G^H G
+                 built from the kernel definition; Cartesian closed form with
+                 scipy's I0; invariances (duplicates, shifts, permutations).
+  sigpy_ref.py   sigpy's KB interpolate / gridding vs the brute-force sum
+                 (widths 1..4); adjointness; == interp_ref for widths >= 2.
 """

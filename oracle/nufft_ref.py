@@ -20,7 +20,7 @@ import torch
 import torch.fft as tfft
 import torch.nn.functional as F
 
-from . import interp_ref
+from . import interp_ref, sigpy_ref
 
 
 def fft_c(x, dim, norm="ortho"):
@@ -171,6 +171,61 @@ class TritonNufftRef:
                 f = fft_c(f, dims) * kerns[n:n + 1, None]
                 out[n:n + 1, i:i + 1] = pad.adjoint(ifft_c(f, dims))
         return out.reshape(img.shape)
+
+
+class SigpyNufftRef(TritonNufftRef):
+    """fourier.py:728-1033 on CPU: the transform of TritonNufftRef with sigpy's
+    interpolate / gridding (``sigpy_ref``; fourier.py:901-963), sigpy_nufft's beta default (1.0 when the formula's radicand is negative,
+    :740-747), its apodization (sinh + 1e-12, continued by sqrt(-a)/sin(sqrt(-a)),
+    :817-841) and Toeplitz kernels from a default-beta instance (:1007-1033)."""
+
+    def __init__(self, im_size, oversamp=1.25, width=6, beta=None, apodize=True, dtype=torch.float32):
+        if beta is None and ((width / oversamp) * (oversamp - 0.5)) ** 2 - 0.8 < 0:
+            beta = 1.0
+        super().__init__(im_size, oversamp, width, beta=beta, apodize=apodize, dtype=dtype)
+
+    def forward_interp_only(self, ksp_os, trj):
+        """fourier.py:901-933: sigpy's interpolate per trajectory, / width^d."""
+        nd = len(self.im_size)
+        t = self._scale_trj(trj, (*ksp_os.shape[:-nd], *self.im_size))
+        outs = []
+        for i in range(trj.shape[0]):
+            g = ksp_os[i]
+            bshape = g.shape[:-nd]
+            y = sigpy_ref.interpolate(g.reshape(-1, *g.shape[-nd:]), t[i].reshape(-1, nd), self.width, self.beta,
+                                      dtype=self.dtype)
+            outs.append(y.reshape(*bshape, *t[i].shape[:-1]))
+        return torch.stack(outs, 0) / self.width ** nd
+
+    def adjoint_grid_only(self, ksp, trj):
+        """fourier.py:935-963: sigpy's gridding per trajectory, / width^d."""
+        nd = trj.shape[-1]
+        N = trj.shape[0]
+        oshape = (N, *ksp.shape[1:-(trj.ndim - 2)], *self.im_size)
+        os_shape = list(oshape)[:-nd] + [ceil(self.oversamp * i) for i in oshape[-nd:]]
+        t = self._scale_trj(trj, oshape)
+        M = math.prod(trj.shape[1:-1])
+        outs = []
+        for i in range(N):
+            y = ksp[i].reshape(-1, M)
+            outs.append(sigpy_ref.gridding(y, t[i].reshape(-1, nd), tuple(os_shape[-nd:]), self.width, self.beta,
+                                           dtype=self.dtype))
+        return torch.stack(outs, 0).reshape(os_shape) / self.width ** nd
+
+    def _apodization_func(self, x):
+        eps = 1e-12
+        arg = self.beta ** 2 - (np.pi * self.width * x) ** 2
+        pos = arg.clamp(min=0).sqrt()
+        pos = pos / (torch.sinh(pos) + eps)
+        neg = (-arg.clamp(max=0)).sqrt()
+        neg = neg / (torch.sin(neg) + eps)
+        return pos + neg
+
+    def calc_teoplitz_kernels(self, trj, weights=None, os_factor=2.0):
+        im_size_os = tuple(round(i * os_factor) for i in self.im_size)
+        nos = SigpyNufftRef(im_size_os, self.oversamp, self.width, dtype=self.dtype)
+        return calc_toep_kernel_helper(nos.adjoint, trj * os_factor, weights, self.cdtype) * (
+            os_factor ** len(self.im_size))
 
 
 def calc_toep_kernel_helper(nufft_adj_os, trj, weights=None, cdtype=torch.complex64):

@@ -71,6 +71,13 @@ def test_cpu_tensors_are_rejected_loudly():
         ungrid(torch.zeros((8, 8), dtype=torch.complex64), torch.zeros((5, 2)), 4.0)
     with pytest.raises(RuntimeError, match="GPU only"):
         grid(torch.zeros(5, dtype=torch.complex64), torch.zeros((5, 2)), (8, 8), 4.0)
+    from mr_recon_amd.algs import density_compensation
+    from mr_recon_amd.fourier import sigpy_nufft
+
+    with pytest.raises(RuntimeError, match="GPU only"):
+        density_compensation(torch.zeros((10, 2)), (8, 8))
+    with pytest.raises(RuntimeError, match="GPU only"):
+        sigpy_nufft((8, 8)).forward(x, k)
 
 
 def test_option_validation_matches_reference_errors():
@@ -123,6 +130,19 @@ def test_beta_and_shapes_of_hip_nufft():
     assert triton_nufft is hip_nufft
     sc, sh = n._scale_shift((150, 100))
     assert sc[0] == float(np.float32(188 / 150)) and sh == [94.0, 62.0]
+
+
+def test_sigpy_nufft_beta_default(capsys):
+    """sigpy_nufft's beta: the KB formula, or 1.0 (with the reference's warning unless
+    width == 1) where its radicand is negative (fourier.py:740-747)."""
+    from mr_recon_amd.fourier import kb_beta, sigpy_nufft
+
+    assert sigpy_nufft((16, 16), oversamp=2.0, width=6).beta == kb_beta(6, 2.0)
+    assert sigpy_nufft((16, 16), oversamp=1.25, width=1).beta == 1.0
+    assert "WARNING" not in capsys.readouterr().out
+    assert sigpy_nufft((16, 16), oversamp=1.0, width=1.5).beta == 1.0
+    assert "WARNING: beta is set to 1.0" in capsys.readouterr().out
+    assert sigpy_nufft((16, 16), beta=3.0).beta == 3.0
 
 
 def test_synthetic_generators():

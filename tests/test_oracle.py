@@ -10,8 +10,8 @@ from scipy.special import i0
 from helpers import cdot, edge_coords, rel_l2
 from mr_recon_amd import synth
 from oracle import bins as obins
-from oracle import interp_ref, kb, ndft
-from oracle.nufft_ref import TritonNufftRef, fft_c, resize
+from oracle import interp_ref, kb, ndft, sigpy_ref
+from oracle.nufft_ref import SigpyNufftRef, TritonNufftRef, fft_c, resize
 from oracle.sense_ref import SenseRef, conjugate_gradient
 
 
@@ -254,3 +254,48 @@ def test_oracle_autograd_matches_fp64_finite_differences():
     an = torch.sum(k.grad * v)
     assert abs(float(an - fd)) < 1e-6 * abs(float(fd))
     assert torch.isfinite(k.grad).all()
+
+
+def test_sigpy_apodization_closed_form():
+    """sigpy_nufft's apodization equals z / sinh(z), z = sqrt(beta^2 - (pi w x)^2), on both
+    sides of beta = pi w |x| (complex z past it: sqrt(-a) / sin(sqrt(-a))), and the
+    triton form where the radicand is positive."""
+    nf = SigpyNufftRef((16,), oversamp=1.25, width=1, dtype=torch.float64)
+    assert nf.beta == 1.0
+    x = torch.linspace(-0.5, 0.5, 101, dtype=torch.float64)
+    z = np.sqrt((nf.beta ** 2 - (np.pi * nf.width * x.numpy()) ** 2).astype(np.complex128))
+    with np.errstate(invalid="ignore"):
+        want = np.where(z == 0, 0.0, z / np.sinh(z)).real
+    assert np.allclose(nf._apodization_func(x).numpy(), want, rtol=1e-9, atol=1e-12)
+    a = SigpyNufftRef((16,), 1.25, 6, dtype=torch.float64)
+    b = TritonNufftRef((16,), 1.25, 6, dtype=torch.float64)
+    assert a.beta == b.beta
+    assert torch.allclose(a._apodization_func(x), b._apodization_func(x), rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("os_,w,bound", [(2.0, 6, 2e-5), (1.25, 6, 8e-4)])
+def test_oracle_sigpy_nufft_vs_ndft(os_, w, bound):
+    im = (24, 24)
+    k = ((np.random.default_rng(3).random((900, 2)) - 0.5) * 24).astype(np.float32)
+    x = synth.random_image(im, seed=4)
+    nf = SigpyNufftRef(im, oversamp=os_, width=w, dtype=torch.float64)
+    y = nf.forward(x[None], torch.from_numpy(k)[None])[0].numpy()
+    ye = ndft.ndft_forward(x.numpy(), k, im)
+    assert np.linalg.norm(y - ye) / np.linalg.norm(ye) < bound
+
+
+@pytest.mark.parametrize("D,G,w", [(1, 9, 1.0), (2, 7, 1.0), (2, 7, 1.5), (2, 6, 4.0), (3, 5, 2.5)])
+def test_sigpy_interp_vs_bruteforce(D, G, w):
+    """sigpy's interpolate (every grid point within w/2, all periodic images) vs the
+    pure-Python brute force; gridding is its exact adjoint."""
+    rng = np.random.default_rng(D * 7 + int(2 * w))
+    k = torch.from_numpy(((rng.random((40, D)) - 0.3) * 2 * G).astype(np.float32))
+    vals = synth.random_image((2, *(G,) * D), seed=5).to(torch.complex128)
+    got = sigpy_ref.interpolate(vals, k, w, 2.5, dtype=torch.float64)
+    ref = interp_ref.ungrid_bruteforce(vals, k, w, 2.5)
+    assert np.linalg.norm(got.numpy() - ref) / np.linalg.norm(ref) < 1e-6
+    y = synth.random_image((2, 40), seed=6).to(torch.complex128)
+    g = sigpy_ref.gridding(y, k, (G,) * D, w, 2.5, dtype=torch.float64)
+    assert abs(cdot(y, got) - cdot(g, vals)) < 1e-10 * abs(cdot(y, got))
+    if w >= 2:  # the reference's torch path masks |u| <= w/2 as well, which only bites below 2
+        assert rel_l2(got, interp_ref.ungrid(vals, k, w, beta=2.5, dtype=torch.float64)) < 1e-6
