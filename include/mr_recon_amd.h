@@ -56,6 +56,10 @@ int mra_bin_items(const int* tile_start, int ntiles, int S, int prob, int* nitem
  * kernels.py:295-393 weights2d/weights3d, norm '1'). */
 int mra_plan_weights(int ndim, int P, const float* tsort, int npts, const int* G, const int* wrap, float width,
                      float beta, float* wsort, int* lbase, void* stream);
+/* Same with one Kaiser-Bessel beta per dimension (betas[ndim]): sigpy's interpolate / gridding
+ * with a per-dimension `param`, as type3_nufft grids on (linops.py:250-251, 311-312). */
+int mra_plan_weights_dims(int ndim, int P, const float* tsort, int npts, const int* G, const int* wrap, float width,
+                          const float* betas, float* wsort, int* lbase, void* stream);
 
 /* ---- interpolation / gridding -----------------------------------------------------------
  * mra_interp replaces ungrid (ungrid.py:34-124, Triton kernels _ungrid1d/2d/3d
@@ -78,6 +82,10 @@ int mra_grid_reduce(int ndim, int P, void* grid, const void* scratch, const void
  * reference counterpart (the reference multiplies ksp by dcf before gridding,
  * linops.py:1208). */
 int mra_geom_set_dcf_plan_order(void* geom_host, int on);
+/* Interpolation output, gridding input and dcf all in sorted (plan) order: position s of a
+ * problem's sample vectors is its s-th sorted sample (no perm gather or scatter). Used by
+ * density_compensation (algs.py:20-106), whose iteration never needs sample order. */
+int mra_geom_set_ksp_plan_order(void* geom_host, int on);
 /* Selects the phase-1 kernel behind mra_grid_partial: 1 (default) = 3D cells of samples
  * summed on the matrix cores (P <= 7), 0 = per-lane LDS read-modify-write. Same scratch
  * output; returns the previous selection. No reference counterpart (one kernel per
@@ -151,10 +159,12 @@ int mra_toeplitz_fused(int ndim, const int* N, const int* G, const int* center, 
 /* ---- density compensation ---------------------------------------------------------------
  * Elementwise steps of the Pipe-Menon iteration behind density_compensation (algs.py:20-106,
  * which calls sigpy.mri.pipe_menon_dcf): w /= |G^H G w| after the gridding and interpolation
- * calls above, and the final dcf /= max(dcf) (algs.py:105). part: mra_dcf_partials() floats. */
+ * calls above, and the final dcf /= max(dcf) (algs.py:105), optionally stored through the plan's
+ * sort permutation (out[perm[i]]) when the iteration ran in plan order. part: mra_dcf_partials()
+ * floats. */
 int mra_dcf_partials(void);
 int mra_dcf_update(const void* ghg, float* w, long long n, void* stream);
-int mra_dcf_normalize(float* w, long long n, float* part, void* stream);
+int mra_dcf_normalize(const float* w, const int* perm, float* out, long long n, float* part, void* stream);
 
 /* ---- conjugate gradient -----------------------------------------------------------------
  * Vector work of conjugate_gradient (algs.py:779-872) with FP64 inner products

@@ -35,12 +35,14 @@ SIGNATURES = {
     "mra_geom_size": (_i, []),
     "mra_set_grid_kernel": (_i, [_i]),
     "mra_geom_set_dcf_plan_order": (_i, [_vp, _i]),
+    "mra_geom_set_ksp_plan_order": (_i, [_vp, _i]),
     "mra_default_tile": (_i, [_i]),
     "mra_grad_samples_per_block": (_i, []),
     "mra_subgrid_points": (_ll, [_i, _i]),
     "mra_max_coil_chunk": (_i, [_i, _i]),
     "mra_geom_fill": (_i, [_vp, _ll, _ll, _ll, _ll, _ll, _i, _i, _i, _ip, _i, _ip, _f, _f, _f, _i, _ll, _i]),
     "mra_plan_weights": (_i, [_i, _i, _vp, _i, _ip, _ip, _f, _f, _vp, _vp, _vp]),
+    "mra_plan_weights_dims": (_i, [_i, _i, _vp, _i, _ip, _ip, _f, _fp, _vp, _vp, _vp]),
     "mra_scan_workspace": (_ll, [_ll]),
     "mra_scan_excl_i32": (_i, [_vp, _ll, _vp, _vp, _vp]),
     "mra_bin_workspace": (_ll, [_i, _ll, _ll, _i, _i]),
@@ -76,7 +78,7 @@ SIGNATURES = {
     "mra_cg_step_seg": (_i, [_vp, _vp, _vp, _vp, _ll, _i, _f, _vp, _vp, _vp]),
     "mra_dcf_partials": (_i, []),
     "mra_dcf_update": (_i, [_vp, _vp, _ll, _vp]),
-    "mra_dcf_normalize": (_i, [_vp, _ll, _vp, _vp]),
+    "mra_dcf_normalize": (_i, [_vp, _vp, _vp, _ll, _vp, _vp]),
 }
 
 
@@ -118,7 +120,14 @@ def require_device(*tensors: torch.Tensor) -> None:
 
 
 def ptr(t: torch.Tensor | None):
-    return None if t is None else _vp(t.data_ptr())
+    """Device address of ``t``'s data. Lazy conjugate / negative views (``x.conj()``)
+    share the data of ``x``, so their address would hand the kernels the unconjugated
+    values: they are refused."""
+    if t is None:
+        return None
+    if t.is_conj() or t.is_neg():
+        raise ValueError("a lazy conj()/neg() view cannot be passed to a kernel; call resolve_conj() first")
+    return _vp(t.data_ptr())
 
 
 def stream_ptr(device: torch.device | None = None):

@@ -149,8 +149,10 @@ def density_compensation(trj: torch.Tensor, im_size: tuple, num_iters: Optional[
 
     Here G is the HIP gridding (with w as its per-sample weight on unit samples) and
     G^H the HIP interpolation over one sample plan of the trajectory; the division
-    and the normalisation are HIP kernels. Only ``method='sigpy'`` exists in the
-    reference (its other branches are commented out).
+    and the normalisation are HIP kernels. The iteration keeps w and G^H G w in the
+    plan's sorted order, so every pass over the samples is contiguous; the final
+    normalisation stores through the sort permutation. Only ``method='sigpy'``
+    exists in the reference (its other branches are commented out).
     """
     from . import ops
     from .plan import SamplePlan
@@ -171,16 +173,19 @@ def density_compensation(trj: torch.Tensor, im_size: tuple, num_iters: Optional[
     dev = trj.device
     st = stream_ptr(dev)
     L = lib()
-    w = torch.ones(M, dtype=torch.float32, device=dev)
+    w = torch.ones(M, dtype=torch.float32, device=dev)  # plan (sorted) order
     if num_iters > 0:
         unit = torch.ones((1, M), dtype=torch.complex64, device=dev)
         grid = torch.empty((1, *im_size), dtype=torch.complex64, device=dev)
         ghg = torch.empty((1, M), dtype=torch.complex64, device=dev)
         scratch = torch.empty(ops.gridding_scratch_elems(plan, 1), dtype=torch.complex64, device=dev)
+        geom = plan.geom(1, ksp_plan_order=True)
         for _ in range(int(num_iters)):
-            ops.gridding(unit, plan, grid, dcf=w, scratch=scratch)
-            ops.interp(grid, plan, out=ghg)
+            ops.gridding_reduce(ops.gridding_partial(unit, plan, 1, dcf=w, scratch=scratch, ksp_plan_order=True),
+                                plan, grid)
+            ops.interp(grid, plan, out=ghg, geom=geom)
             check(L.mra_dcf_update(ptr(ghg), ptr(w), M, st))
     part = torch.empty(L.mra_dcf_partials(), dtype=torch.float32, device=dev)
-    check(L.mra_dcf_normalize(ptr(w), M, ptr(part), st))
-    return w.reshape(trj.shape[:-1])
+    out = torch.empty(M, dtype=torch.float32, device=dev)
+    check(L.mra_dcf_normalize(ptr(w), ptr(plan.perm), ptr(out), M, ptr(part), st))
+    return out.reshape(trj.shape[:-1])

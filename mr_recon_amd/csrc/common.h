@@ -39,7 +39,15 @@ struct Geom {
   long long scratch_off;  // float2 offset of this problem's gridding partials
   int tile_items_off;     // offset of this problem's per-tile item table
   int dcf_plan_order;     // 1: the gridding dcf is indexed by sorted (plan) position, 0: by sample
+  int ksp_plan_order;     // 1: k-space (interp output, gridding input) and dcf are in sorted order
+                          //    (interp and gridding only; the trajectory gradient ignores it)
 };
+
+// Position of sorted sample gs in the problem's sample-indexed vectors (k-space, dcf):
+// perm[gs], or the sorted position itself when the caller keeps them in plan order.
+__device__ __forceinline__ int sample_slot(const int* __restrict__ perm, const Geom& g, long long gs) {
+  return g.ksp_plan_order ? (int)(gs - g.samp_off) : perm[gs];
+}
 
 // Work item: one tile's slice of sorted samples.
 //   x = problem index, y = linear tile id (-1: non-finite samples),

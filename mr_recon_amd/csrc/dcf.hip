@@ -57,8 +57,11 @@ __global__ __launch_bounds__(kThreads) void max_partial_kernel(const float* __re
   }
 }
 
-// w[i] /= max(part[0..nb)); each block folds the nb partials itself (nb <= 1024)
-__global__ __launch_bounds__(kThreads) void scale_kernel(float* __restrict__ w, long long n,
+// out[perm ? perm[i] : i] = w[i] / max(part[0..nb)); each block folds the nb partials
+// itself (nb <= 1024)
+// (w and out may alias when perm is null: each element is read and written by one thread)
+__global__ __launch_bounds__(kThreads) void scale_kernel(const float* w, const int* __restrict__ perm, float* out,
+                                                         long long n,
                                                          const float* __restrict__ part, int nb) {
   __shared__ float mx;
   if (threadIdx.x < 64) {
@@ -70,7 +73,7 @@ __global__ __launch_bounds__(kThreads) void scale_kernel(float* __restrict__ w, 
   __syncthreads();
   const float d = mx;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    w[i] = w[i] / d;
+    out[perm ? perm[i] : i] = w[i] / d;
 }
 
 }  // namespace dcf
@@ -91,15 +94,17 @@ int mra_dcf_update(const void* ghg, float* w, long long n, void* stream) {
   MRA_TRY_END
 }
 
-// part: mra_dcf_partials() floats of workspace
-int mra_dcf_normalize(float* w, long long n, float* part, void* stream) {
+// out[perm ? perm[i] : i] = w[i] / max(w); perm (a permutation of 0..n-1) may be null,
+// out may equal w only when perm is null. part: mra_dcf_partials() floats of workspace.
+int mra_dcf_normalize(const float* w, const int* perm, float* out, long long n, float* part, void* stream) {
   MRA_TRY_BEGIN
   if (n <= 0) MRA_FAIL("dcf_normalize: n must be positive");
-  if (!w || !part) MRA_FAIL("dcf_normalize: null pointer");
+  if (!w || !out || !part) MRA_FAIL("dcf_normalize: null pointer");
+  if (perm && out == w) MRA_FAIL("dcf_normalize: a permuted result cannot overwrite its input");
   hipStream_t st = (hipStream_t)stream;
   const unsigned nb = dcf::blocks_for(n);
   dcf::max_partial_kernel<<<nb, dcf::kThreads, 0, st>>>(w, n, part);
-  dcf::scale_kernel<<<nb, dcf::kThreads, 0, st>>>(w, n, part, (int)nb);
+  dcf::scale_kernel<<<nb, dcf::kThreads, 0, st>>>(w, perm, out, n, part, (int)nb);
   MRA_CHECK_LAUNCH("dcf_normalize");
   MRA_TRY_END
 }

@@ -230,7 +230,7 @@ __global__ __launch_bounds__(64 * MfmaGrid<P>::NW) void grid_mfma_kernel(
   int rbase = 0;
   float rw[3 * P];
   float2 ry[M::NCW];
-  int onext = lane < cnt ? perm[gs0 + lane] : 0;
+  int onext = lane < cnt ? sample_slot(perm, g, gs0 + lane) : 0;
   auto gather = [&](int s0) {
     const bool ok = s0 + lane < cnt;
     const long long gs = gs0 + (ok ? s0 + lane : 0);
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(64 * MfmaGrid<P>::NW) void grid_mfma_kernel(
       const float2 v = (ok && c < nc) ? kc[(long long)c * g.ksp_stride] : make_float2(0.f, 0.f);
       ry[c] = make_float2(v.x * sc, v.y * sc);
     }
-    onext = s0 + CH + lane < cnt ? perm[gs0 + s0 + CH + lane] : 0;
+    onext = s0 + CH + lane < cnt ? sample_slot(perm, g, gs0 + s0 + CH + lane) : 0;
   };
   gather(0);
 

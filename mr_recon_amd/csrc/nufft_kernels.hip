@@ -123,10 +123,16 @@ __device__ __forceinline__ void load_weights(const float* __restrict__ wsort, lo
 }
 
 // --------------------------------------------------------------------------------------------
+// Kaiser-Bessel shape per dimension (all equal except for sigpy-style per-dim kernels).
+struct DimBetas {
+  float b[kMaxDim];
+};
+
 // Per sorted sample: KB weights along each dim (unscaled) and the subgrid base offset.
 template <int D, int P, int T>
 __global__ __launch_bounds__(256) void plan_weights_kernel(const float* __restrict__ tsort, int npts, Geom g,
-                                                           float* __restrict__ wsort, int* __restrict__ lbase) {
+                                                           DimBetas bt, float* __restrict__ wsort,
+                                                           int* __restrict__ lbase) {
   constexpr int SUB = T + P - 1;
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= npts) return;
@@ -137,7 +143,7 @@ __global__ __launch_bounds__(256) void plan_weights_kernel(const float* __restri
   for (int d = 0; d < D; ++d) {
     const float t = tsort[(long long)s * D + d];
     ok = ok && isfinite(t);
-    const int lower = ok ? kb_weights<P>(t, g.r, g.beta, 1.0f, w) : 0;
+    const int lower = ok ? kb_weights<P>(t, g.r, bt.b[d], 1.0f, w) : 0;
     const int lw = ok ? pmod(lower + g.wrap[d], g.G[d]) % T : 0;
     base = base * SUB + lw;
 #pragma unroll
@@ -172,7 +178,7 @@ __global__ __launch_bounds__(kThreads) void interp_kernel(const float2* __restri
 
   if (it.y < 0) {  // samples with non-finite coordinates: defined as 0
     for (int s = it.z + tid; s < it.z + it.w; s += kThreads) {
-      const int o = perm[g.samp_off + s];
+      const int o = sample_slot(perm, g, g.samp_off + s);
       for (int cl = 0; cl < nc; ++cl) odst[(long long)cl * g.ksp_stride + o] = make_float2(0.f, 0.f);
     }
     return;
@@ -197,7 +203,7 @@ __global__ __launch_bounds__(kThreads) void interp_kernel(const float2* __restri
     float wx[P], wr[TS::PR];
     load_weights<D, P>(wsort, gs, g.out_scale, wx, wr);
     const int base = lbase[gs];
-    const int o = perm[gs];
+    const int o = sample_slot(perm, g, gs);
     for (int cl = 0; cl < nc; ++cl) {
       const float2* sb = lds2 + cl * TS::SUBN + base;
       float ax = 0.f, ay = 0.f;
@@ -250,7 +256,7 @@ __global__ __launch_bounds__(kThreads) void interp_pair_kernel(const float2* __r
 
   if (it.y < 0) {  // samples with non-finite coordinates: defined as 0
     for (int s = it.z + tid; s < it.z + it.w; s += kThreads) {
-      const int o = perm[g.samp_off + s];
+      const int o = sample_slot(perm, g, g.samp_off + s);
       for (int cl = 0; cl < nc; ++cl) odst[(long long)cl * g.ksp_stride + o] = make_float2(0.f, 0.f);
     }
     return;
@@ -280,7 +286,7 @@ __global__ __launch_bounds__(kThreads) void interp_pair_kernel(const float2* __r
     float wx[P], wr[TS::PR];
     load_weights<D, P>(wsort, gs, g.out_scale, wx, wr);
     const int base = lbase[gs];
-    const int o = perm[gs];
+    const int o = sample_slot(perm, g, gs);
     for (int cp = 0; cp < npair; ++cp) {
       const float4* sb = lds4 + cp * TS::SUBN + base;
       float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
@@ -605,10 +611,10 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
   StageRegs<D, P, RL::NC> sr;
   {
     const bool ok = lane < CH && lane < cnt;
-    const int o = ok ? perm[gs0 + lane] : 0;
+    const int o = ok ? sample_slot(perm, g, gs0 + lane) : 0;
     stage_load<D, P, T>(sr, ok, gs0 + lane, o, ksp, dcf, wsort, lbase, g, c0, nc);
   }
-  int onext = (lane < CH && CH + lane < cnt) ? perm[gs0 + CH + lane] : 0;
+  int onext = (lane < CH && CH + lane < cnt) ? sample_slot(perm, g, gs0 + CH + lane) : 0;
 
   for (int e = lane; e < RL::LDS_FLOAT2; e += 64) lds2[e] = make_float2(0.f, 0.f);
 
@@ -679,7 +685,7 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
     if (s0 + CH < cnt) {
       const int sl = s0 + CH + lane;
       stage_load<D, P, T>(sr, lane < CH && sl < cnt, gs0 + sl, onext, ksp, dcf, wsort, lbase, g, c0, nc);
-      onext = (lane < CH && sl + CH < cnt) ? perm[gs0 + sl + CH] : 0;
+      onext = (lane < CH && sl + CH < cnt) ? sample_slot(perm, g, gs0 + sl + CH) : 0;
     }
     const int n = min(CH, cnt - s0);
     {
@@ -1200,6 +1206,7 @@ struct LaunchArgs {
   int prob, nprob, ntiles, nbatch, accumulate;
   const int* tile_items;
   int npts;
+  DimBetas betas;       // Weights: KB shape per dimension
   hipStream_t stream;
 };
 
@@ -1220,7 +1227,7 @@ static int launch_op(const LaunchArgs& a) {
     case Op::Weights:
       if (a.npts > 0)
         plan_weights_kernel<D, P, T><<<(a.npts + 255) / 256, 256, 0, a.stream>>>(
-            (const float*)a.src, a.npts, a.geom, (float*)a.dst, (int*)a.lbase);
+            (const float*)a.src, a.npts, a.geom, a.betas, (float*)a.dst, (int*)a.lbase);
       break;
     case Op::Interp: {
       if (a.nitems <= 0) break;
@@ -1396,12 +1403,21 @@ int mra_geom_set_dcf_plan_order(void* geom_host, int on) {
   MRA_TRY_END
 }
 
+int mra_geom_set_ksp_plan_order(void* geom_host, int on) {
+  MRA_TRY_BEGIN
+  if (!geom_host) MRA_FAIL("geom: null record");
+  ((Geom*)geom_host)->ksp_plan_order = on ? 1 : 0;
+  MRA_TRY_END
+}
+
 // Per sorted sample KB weights (npts x ndim x P floats) and subgrid offsets.
-int mra_plan_weights(int ndim, int P, const float* tsort, int npts, const int* G, const int* wrap, float width,
-                     float beta, float* wsort, int* lbase, void* stream) {
+int mra_plan_weights_dims(int ndim, int P, const float* tsort, int npts, const int* G, const int* wrap, float width,
+                          const float* betas, float* wsort, int* lbase, void* stream) {
   MRA_TRY_BEGIN
   if (ndim < 1 || ndim > 3) MRA_FAIL("plan_weights: ndim must be 1..3");
+  if (!betas) MRA_FAIL("plan_weights: null betas");
   LaunchArgs a{};
+  for (int d = 0; d < kMaxDim; ++d) a.betas.b[d] = betas[d < ndim ? d : 0];
   a.op = Op::Weights;
   a.src = tsort;
   a.dst = wsort;
@@ -1412,10 +1428,16 @@ int mra_plan_weights(int ndim, int P, const float* tsort, int npts, const int* G
     a.geom.wrap[d] = (d < ndim && wrap) ? wrap[d] : 0;
   }
   a.geom.r = 0.5f * width;
-  a.geom.beta = beta;
+  a.geom.beta = betas[0];
   a.stream = (hipStream_t)stream;
   return dispatch(ndim, P, a);
   MRA_TRY_END
+}
+
+int mra_plan_weights(int ndim, int P, const float* tsort, int npts, const int* G, const int* wrap, float width,
+                     float beta, float* wsort, int* lbase, void* stream) {
+  const float b[kMaxDim] = {beta, beta, beta};
+  return mra_plan_weights_dims(ndim, P, tsort, npts, G, wrap, width, b, wsort, lbase, stream);
 }
 
 int mra_interp(int ndim, int P, const void* grid, void* out, const float* wsort, const int* lbase, const int* perm,

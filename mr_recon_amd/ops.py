@@ -19,6 +19,7 @@ from .plan import SamplePlan
 def _c64(t: torch.Tensor, name: str) -> torch.Tensor:
     if t.dtype != torch.complex64:
         raise TypeError(f"{name} must be complex64, got {t.dtype}")
+    t = t.resolve_conj().resolve_neg()  # inputs: materialise lazy conj()/neg() views
     if not t.is_contiguous():
         t = t.contiguous()
     return t
@@ -70,10 +71,11 @@ def _grid_batch(plan: SamplePlan, out: torch.Tensor) -> int:
 
 def gridding_partial(ksp: torch.Tensor, plan: SamplePlan, nbatch: int, dcf: torch.Tensor | None = None,
                      out_scale: float = 1.0, scratch: torch.Tensor | None = None,
-                     dcf_plan_order: bool = False) -> torch.Tensor:
+                     dcf_plan_order: bool = False, ksp_plan_order: bool = False) -> torch.Tensor:
     """Gridding phase 1: every work item's samples summed into its subgrids in
     ``scratch`` (returned; allocated when not given). ``dcf_plan_order``: ``dcf`` is
-    given in sorted (plan) order, ``dcf[plan.perm]``, and is read contiguously."""
+    given in sorted (plan) order, ``dcf[plan.perm]``, and is read contiguously;
+    ``ksp_plan_order``: ``ksp`` and ``dcf`` both are."""
     _lib.require_device(ksp, dcf)
     ksp = _c64(ksp, "ksp")
     B = int(nbatch)
@@ -83,7 +85,8 @@ def gridding_partial(ksp: torch.Tensor, plan: SamplePlan, nbatch: int, dcf: torc
         if dcf.dtype != torch.float32 or dcf.numel() != plan.npts:
             raise ValueError("dcf must be float32 with one entry per sample")
         dcf = dcf.contiguous()
-    g = plan.geom(B, plan.npts, out_scale, dcf_plan_order=dcf is not None and dcf_plan_order)
+    g = plan.geom(B, plan.npts, out_scale, dcf_plan_order=dcf is not None and dcf_plan_order,
+                  ksp_plan_order=ksp_plan_order)
     need = gridding_scratch_elems(plan, B)
     if scratch is None:
         scratch = torch.empty(need, dtype=torch.complex64, device=ksp.device)
@@ -119,6 +122,8 @@ def interp_grad(grid: torch.Tensor, a: torch.Tensor, plan: SamplePlan, out_scale
     gt[m, d] = sum_b Re( conj(a[b, m]) * sum_p dW_p/dt_d * grid[b, g_p] )
     """
     _lib.require_device(grid, a)
+    if len(set(plan.betas)) > 1:
+        raise NotImplementedError("interp_grad: plans with one beta per dimension are not supported")
     grid = _c64(grid, "grid")
     a = _c64(a, "a")
     D = plan.ndim

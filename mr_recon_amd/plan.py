@@ -47,6 +47,7 @@ class GeomKey:
     out_scale: float
     grid_stride: int
     dcf_plan_order: bool = False
+    ksp_plan_order: bool = False
 
 
 class SamplePlan:
@@ -57,7 +58,7 @@ class SamplePlan:
     coords : (..., D) float32 device tensor, reference units (see ``scale``)
     grid_size : tuple of D ints, the (oversampled) grid the samples live on
     width : kernel width in grid points (same in every dim)
-    beta : Kaiser-Bessel shape parameter
+    beta : Kaiser-Bessel shape parameter (or one per dimension)
     scale, shift : per-dim fp32 scale and shift applied before wrapping (the
         NUFFT's ``_scale_trj``); ``None`` means coordinates are already in grid
         units (the plain ``ungrid``/``grid`` surface)
@@ -85,7 +86,13 @@ class SamplePlan:
         self.P = kernel_points(width)
         if not 2 <= self.P <= 9:
             raise ValueError(f"kernel width {width} unsupported: floor(width)+1 must be in 2..9")
-        self.beta = float(beta)
+        if isinstance(beta, (tuple, list)):  # one KB shape per dimension (sigpy's `param` tuple)
+            if len(beta) != D:
+                raise ValueError(f"{len(beta)} betas for a {D}-dimensional grid")
+            self.betas = tuple(float(b) for b in beta)
+        else:
+            self.betas = (float(beta),) * D
+        self.beta = self.betas[0]
         self.coords_shape = tuple(coords.shape[:-1])
         self.npts = int(math.prod(self.coords_shape))
         if self.npts <= 0:
@@ -127,8 +134,9 @@ class SamplePlan:
         # KB weights (D x P per sorted sample) and subgrid offsets, once per trajectory
         self.wsort = torch.empty(self.npts * D * self.P, dtype=torch.float32, device=dev)
         self.lbase = torch.empty(self.npts, dtype=torch.int32, device=dev)
-        check(L.mra_plan_weights(D, self.P, ptr(self.tsort), self.npts, _lib.ints(self.grid_size),
-                                 _lib.ints(self.wrap), self.width, self.beta, ptr(self.wsort), ptr(self.lbase), st))
+        check(L.mra_plan_weights_dims(D, self.P, ptr(self.tsort), self.npts, _lib.ints(self.grid_size),
+                                      _lib.ints(self.wrap), self.width, _lib.floats(self.betas), ptr(self.wsort),
+                                      ptr(self.lbase), st))
         # The only host synchronisation of a plan: the number of work items, which
         # sizes the launches and the gridding scratch.
         self.nitems = int(item_off[nkeys].item())
@@ -144,19 +152,21 @@ class SamplePlan:
         return max(1, min(nbatch, self.max_coil_chunk))
 
     def geom(self, nbatch: int, ksp_stride: int | None = None, out_scale: float = 1.0,
-             grid_stride: int | None = None, dcf_plan_order: bool = False) -> torch.Tensor:
+             grid_stride: int | None = None, dcf_plan_order: bool = False,
+             ksp_plan_order: bool = False) -> torch.Tensor:
         """Device Geom table (one entry) for a call on ``nbatch`` grids; ``dcf_plan_order``:
-        the gridding dcf comes in sorted (plan) order (``dcf[perm]``)."""
+        the gridding dcf comes in sorted (plan) order (``dcf[perm]``); ``ksp_plan_order``:
+        k-space (interp output, gridding input) and dcf are all in sorted order."""
         gstride = int(grid_stride if grid_stride is not None else math.prod(self.grid_size))
         kstride = int(ksp_stride if ksp_stride is not None else self.npts)
-        key = GeomKey(int(nbatch), kstride, float(out_scale), gstride, bool(dcf_plan_order))
+        key = GeomKey(int(nbatch), kstride, float(out_scale), gstride, bool(dcf_plan_order), bool(ksp_plan_order))
         g = self._geoms.get(key)
         if g is None:
             g = make_geoms([dict(grid_off=0, grid_stride=gstride, ksp_off=0, ksp_stride=kstride, samp_off=0,
                                  nbatch=int(nbatch), npts=self.npts, grid_size=self.grid_size, tile=self.tile,
                                  wrap=self.wrap, beta=self.beta, width=self.width, out_scale=out_scale,
                                  item_base=0, scratch_off=0, tile_items_off=0,
-                                 dcf_plan_order=bool(dcf_plan_order))],
+                                 dcf_plan_order=bool(dcf_plan_order), ksp_plan_order=bool(ksp_plan_order))],
                            self.device)
             self._geoms[key] = g
         return g
@@ -175,5 +185,7 @@ def make_geoms(entries: list[dict], device) -> torch.Tensor:
                               e.get("item_base", 0), e.get("scratch_off", 0), e.get("tile_items_off", 0)))
         if e.get("dcf_plan_order"):
             check(L.mra_geom_set_dcf_plan_order(C.addressof(buf) + i * gsz, 1))
+        if e.get("ksp_plan_order"):
+            check(L.mra_geom_set_ksp_plan_order(C.addressof(buf) + i * gsz, 1))
     host = torch.frombuffer(bytearray(bytes(buf)), dtype=torch.uint8)
     return host.to(device)

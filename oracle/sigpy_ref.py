@@ -34,7 +34,9 @@ def _patch(coords, width, beta, dtype):
     x = lower[:, None, :] + offs[None]                      # (M, P^D, D)
     u = (x.to(dtype) - k[:, None, :]) / r
     inside = u.abs() <= 1
-    kb = kaiser_bessel_torch(torch.where(inside, u, torch.zeros_like(u)), beta)
+    # beta: a number, or one per dimension (sigpy's `param` tuple)
+    b = torch.as_tensor(beta, dtype=dtype) if isinstance(beta, (tuple, list)) else beta
+    kb = kaiser_bessel_torch(torch.where(inside, u, torch.zeros_like(u)), b)
     w = torch.where(inside, kb, torch.zeros_like(kb)).prod(-1)
     return x, w
 

@@ -116,3 +116,32 @@ def test_dcf_in_plan_order(dev, D):
                                                      dcf_plan_order=True), plan, b)
         assert rel_l2(a, ref) < TOL
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("D,C", [(2, 1), (2, 4), (3, 1), (3, 5)])
+def test_ksp_in_plan_order(dev, D, C):
+    """Interpolation output and gridding input (and dcf) kept in sorted (plan) order give
+    bit-identical values at the permuted positions, for the single- and coil-pair
+    interpolation kernels and both phase-1 gridding kernels."""
+    from mr_recon_amd import ops
+    from mr_recon_amd.plan import SamplePlan
+
+    G = 32
+    k = torch.from_numpy(edge_coords((G,) * D, 4000, seed=D + C, width=4.0))
+    gen = torch.Generator().manual_seed(C)
+    grid = torch.randn((C, *(G,) * D), dtype=torch.complex64, generator=gen).to(dev)
+    y = torch.randn((C, k.shape[0]), dtype=torch.complex64, generator=gen).to(dev)
+    dcf = (torch.rand(k.shape[0], generator=gen) + 0.5).to(dev)
+    plan = SamplePlan(k.to(dev), (G,) * D, 4.0, 9.0)
+    perm = plan.perm.long()
+    a = ops.interp(grid, plan)
+    b = ops.interp(grid, plan, geom=plan.geom(C, ksp_plan_order=True))
+    assert torch.equal(a[:, perm], b)
+    for mode in (1, 0):
+        with grid_kernel(mode):
+            ga = torch.empty((C, *(G,) * D), dtype=torch.complex64, device=dev)
+            gb = torch.empty_like(ga)
+            ops.gridding_reduce(ops.gridding_partial(y, plan, C, dcf=dcf), plan, ga)
+            ops.gridding_reduce(ops.gridding_partial(y[:, perm].contiguous(), plan, C, dcf=dcf[perm].contiguous(),
+                                                     ksp_plan_order=True), plan, gb)
+        assert torch.equal(ga, gb)

@@ -155,3 +155,18 @@ def test_synthetic_generators():
     assert torch.allclose(rss, torch.ones_like(rss), atol=1e-5)
     s = synth.spiral_2d(4, 100, 64)
     assert float(s.abs().max()) < 32
+
+
+def test_lazy_conj_views_are_refused():
+    """x.conj() shares x's memory with a conjugate bit; its raw address would hand a
+    kernel the unconjugated values, so the binding refuses it (inputs of the ops
+    materialise it first)."""
+    from mr_recon_amd import _lib
+    from mr_recon_amd.ops import _c64
+
+    x = torch.randn(4, dtype=torch.complex64)
+    with pytest.raises(ValueError, match="resolve_conj"):
+        _lib.ptr(x.conj())
+    y = _c64(x.conj(), "x")
+    assert not y.is_conj() and torch.equal(y, torch.conj_physical(x))
+    assert _lib.ptr(x) is not None
