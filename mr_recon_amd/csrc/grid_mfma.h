@@ -48,14 +48,14 @@ namespace mra {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-template <int P>
+template <int P, int NWV = MRA_MFMA_NW>
 struct MfmaGrid {
   static constexpr int T = 8;                  // 3D tile edge
   static constexpr int SUB = T + P - 1;        // subgrid edge
   static constexpr int PP = SUB * SUB;         // dense points per plane
   static constexpr int SUBN = SUB * PP;
   static constexpr int NCW = MRA_MFMA_NCW;     // coils per wave
-  static constexpr int NW = MRA_MFMA_NW;       // waves per block
+  static constexpr int NW = NWV;               // waves per block
   static constexpr int NC = NW * NCW;          // coils per block
   static constexpr int PV = 8 / NCW;           // values of p2 per 16 rows
   static constexpr int PF = 2 * NCW;           // floats per grid point in the ring
@@ -121,9 +121,9 @@ __device__ __forceinline__ void mfma_retire(int lo, int hi, int nc, float* ring,
 // B operand columns W*CPW .. (W+1)*CPW-1 of this lane's sample: column j = (p0, p1) =
 // (j / E1P, j % E1P) holds w0[p0] * w1[p1 - off1] (0 outside the patch or past NJ).
 // Layout [cb][sample][16]; four columns per 128-bit store.
-template <int P, int W>
+template <int P, int W, int NWV>
 __device__ __forceinline__ void mfma_stage_b(const float (&rw)[3 * P], int off1, float* sbw, int lane) {
-  using M = MfmaGrid<P>;
+  using M = MfmaGrid<P, NWV>;
 #pragma unroll
   for (int c4 = 0; c4 < M::CPW / 4; ++c4) {
     float v[4];
@@ -185,12 +185,12 @@ struct CellFlush {
   }
 };
 
-template <int P>
-__global__ __launch_bounds__(64 * MfmaGrid<P>::NW) void grid_mfma_kernel(
+template <int P, int NWV>
+__global__ __launch_bounds__(64 * NWV) void grid_mfma_kernel(
     const float2* __restrict__ ksp, const float* __restrict__ dcf, const float* __restrict__ wsort,
     const int* __restrict__ lbase, const int* __restrict__ perm, const Geom* __restrict__ geoms,
     const Item* __restrict__ items, int nitems, int ngroups, float2* __restrict__ scratch) {
-  using M = MfmaGrid<P>;
+  using M = MfmaGrid<P, NWV>;
   constexpr int SUB = M::SUB, CH = M::CH, RB = M::RB, CB = M::CB, PF = M::PF;
   __shared__ __attribute__((aligned(16))) float ring_all[M::NW * M::RING];
   __shared__ __attribute__((aligned(16))) float sbw[CB * CH * 16];         // B operands [cb][s][16]
@@ -272,10 +272,10 @@ __global__ __launch_bounds__(64 * MfmaGrid<P>::NW) void grid_mfma_kernel(
       key = (l0 << 16) | ((l1 / M::E1) << 8) | (l2 / M::E2);
       const int off1 = l1 % M::E1, off2 = l2 % M::E2;
       switch (wave) {
-        case 0: mfma_stage_b<P, 0>(rw, off1, sbw, lane); break;
-        case 1: mfma_stage_b<P, 1 % M::NW>(rw, off1, sbw, lane); break;
-        case 2: mfma_stage_b<P, 2 % M::NW>(rw, off1, sbw, lane); break;
-        default: mfma_stage_b<P, 3 % M::NW>(rw, off1, sbw, lane); break;
+        case 0: mfma_stage_b<P, 0, NWV>(rw, off1, sbw, lane); break;
+        case 1: mfma_stage_b<P, 1 % M::NW, NWV>(rw, off1, sbw, lane); break;
+        case 2: mfma_stage_b<P, 2 % M::NW, NWV>(rw, off1, sbw, lane); break;
+        default: mfma_stage_b<P, 3 % M::NW, NWV>(rw, off1, sbw, lane); break;
       }
       if (wave == M::NW - 1) {
         // dim-2 weights on the region: cw[p2] = w2[p2 - off2], stored by p2 mod PV

@@ -1248,9 +1248,17 @@ static int launch_op(const LaunchArgs& a) {
       if (a.nitems <= 0) break;
       if constexpr (D == 3 && P >= 2 && P <= 7) {
         if (g_grid_kernel == 1) {
+          // A batch that fits one wave's coils runs one wave per block: a second wave
+          // would have no coils but hold half the block's LDS, halving residency.
+          if (a.nbatch <= MfmaGrid<P, 1>::NC) {
+            grid_mfma_kernel<P, 1><<<item_grid(a.nitems, 1), 64, 0, a.stream>>>(
+                (const float2*)a.src, (const float*)a.aux, a.wsort, a.lbase, a.perm, a.geoms, a.items, a.nitems, 1,
+                (float2*)a.dst);
+            break;
+          }
           using M = MfmaGrid<P>;
           const int ng = (a.nbatch + M::NC - 1) / M::NC;
-          grid_mfma_kernel<P><<<item_grid(a.nitems, ng), 64 * M::NW, 0, a.stream>>>(
+          grid_mfma_kernel<P, M::NW><<<item_grid(a.nitems, ng), 64 * M::NW, 0, a.stream>>>(
               (const float2*)a.src, (const float*)a.aux, a.wsort, a.lbase, a.perm, a.geoms, a.items, a.nitems, ng,
               (float2*)a.dst);
           break;

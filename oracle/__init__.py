@@ -27,4 +27,7 @@ How each part is pinned (tests/test_oracle_*.py):
                  scipy's I0; invariances (duplicates, shifts, permutations).
   sigpy_ref.py   sigpy's KB interpolate / gridding vs the brute-force sum
                  (widths 1..4); adjointness; == interp_ref for widths >= 2.
+  type3_ref.py   type-3 NUFFT vs the reference's exact-sum class
+                 (type3_nufft_naive) within the KB error; adjointness;
+                 Toeplitz normal == adjoint(forward).
 """
