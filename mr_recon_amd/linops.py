@@ -1,5 +1,6 @@
-"""Linear operators on the NUFFT path: ``linop`` base, ``batching_params``, ``sense_linop``
-and the type-3 NUFFT (``type3_nufft``, ``type3_nufft_naive``).
+"""Linear operators on the NUFFT path: ``linop`` base, ``batching_params``, ``sense_linop``,
+the subspace model (``subspace_linop``) and the type-3 NUFFT (``type3_nufft``,
+``type3_nufft_naive``).
 
 Mirrors the reference (linops.py:22-47, 1041-1308) call for call. With the HIP
 NUFFT and a single imperfection segment (the default), every SENSE
@@ -27,7 +28,7 @@ import torch.nn as nn
 
 from . import _lib, ops
 from .dtypes import complex_dtype, real_dtype
-from .fourier import NUFFT, hip_nufft, sigpy_nufft, toeplitz_unshifted
+from .fourier import NUFFT, fft, hip_nufft, ifft, sigpy_nufft, toeplitz_unshifted
 from .pad import PadLast
 from .plan import SamplePlan
 from .utils import batch_iterator
@@ -390,4 +391,103 @@ class sense_linop(linop):
                     f = padder.adjoint(ifft(mf, dim=dims))
                     b = (f * self.spatial_funcs[m1:m2].conj()).sum(dim=-d - 1)
                     out += (b * mps.conj()).sum(dim=0)
+        return out
+
+
+class subspace_linop(linop):
+    """Subspace SENSE model (linops.py:1496-1772): the k-space of coil c at time point t is
+    sum_l T_l(t) sum_k phi_k(t) F(S_c B_l alpha_k), with phi (nsub, ntr) indexed by the
+    trajectory's last batch axis, alpha (nsub, *im_size) the coefficient images and
+    optional imperfection segments B_l (spatial) / T_l (temporal).
+
+    All coils, coefficients and segments go through one HIP NUFFT call per application
+    (the reference loops over coil / coefficient / segment batches); the contraction
+    with phi and the segment sums are elementwise torch work on the NUFFT's output. The
+    Toeplitz normal builds the nsub x nsub PSF kernels of conj(phi_k1) phi_k2 dcf as the
+    reference does (:1563-1580) and mixes the coefficient images through them.
+    ``nufft`` defaults to ``hip_nufft`` (the reference defaults to torchkbnufft).
+    """
+
+    def __init__(self, im_size: tuple, trj: torch.Tensor, mps: torch.Tensor, phi: torch.Tensor,
+                 dcf: Optional[torch.Tensor] = None, nufft: Optional[NUFFT] = None,
+                 use_toeplitz: Optional[bool] = False, spatial_funcs: Optional[torch.Tensor] = None,
+                 temporal_funcs: Optional[torch.Tensor] = None, bparams: Optional[batching_params] = None):
+        ishape = (phi.shape[0], *im_size)
+        oshape = (mps.shape[0], *trj.shape[:-1])
+        super().__init__(ishape, oshape)
+        torch_dev = trj.device
+        assert phi.device == torch_dev
+        assert mps.device == torch_dev
+        _lib.require_device(trj, mps, phi)
+        if nufft is None:
+            nufft = hip_nufft(im_size)
+        if dcf is None:
+            dcf = torch.ones(trj.shape[:-1], dtype=real_dtype, device=torch_dev)
+        else:
+            assert dcf.device == torch_dev
+        if spatial_funcs is None:
+            spatial_funcs = torch.ones((1,) * len(ishape), dtype=complex_dtype, device=torch_dev)
+        if temporal_funcs is None:
+            temporal_funcs = torch.ones((1,) * len(oshape), dtype=complex_dtype, device=torch_dev)
+        trj = nufft.rescale_trajectory(trj).type(real_dtype)
+        self.im_size = tuple(im_size)
+        self.trj_size = tuple(trj.shape[:-1])
+        self.use_toeplitz = use_toeplitz
+        self.trj = trj
+        self.phi = phi.type(complex_dtype)
+        self.mps = mps.type(complex_dtype)
+        self.dcf = dcf.type(real_dtype)
+        self.nufft = nufft
+        self.bparams = bparams if bparams is not None else batching_params()
+        self.torch_dev = torch_dev
+        self.spatial_funcs = spatial_funcs.type(complex_dtype)
+        self.temporal_funcs = temporal_funcs.type(complex_dtype)
+        if use_toeplitz:
+            if math.prod(spatial_funcs.shape) > 1:
+                raise NotImplementedError
+            K = self.phi.shape[0]
+            pp = self.phi.conj()[:, None, :] * self.phi[None]                   # nsub nsub ntr
+            w = pp.reshape(K * K, *([1] * (len(self.trj_size) - 1)), pp.shape[-1]) * self.dcf
+            kerns = nufft.calc_teoplitz_kernels(trj[None], w[None])[0]         # (K K) *im_os
+            self.toep_kerns = kerns.reshape(K, K, *kerns.shape[1:])
+        else:
+            self.toep_kerns = None
+
+    def _phi_b(self) -> torch.Tensor:
+        """phi (nsub, ntr) broadcast against (..., nsub, nseg, *trj_size)."""
+        K = self.phi.shape[0]
+        return self.phi.reshape(K, 1, *([1] * (len(self.trj_size) - 1)), -1)
+
+    def forward(self, alphas: torch.Tensor) -> torch.Tensor:
+        """alphas (nsub, *im_size) -> ksp (nc, *trj_size)."""
+        d = len(self.im_size)
+        mw = self.mps[:, None] * self.spatial_funcs                                 # nc nseg *im
+        sx = mw[:, None] * alphas.type(complex_dtype)[None, :, None]                 # nc nsub nseg *im
+        nc, K, L = sx.shape[:3]
+        fsx = self.nufft.forward(sx.reshape(1, nc, K, L, *self.im_size), self.trj[None])[0]   # nc nsub nseg *trj
+        pfsx = (fsx * self._phi_b()).sum(dim=1)                                      # nc nseg *trj
+        return (pfsx * self.temporal_funcs).sum(dim=1)
+
+    def adjoint(self, ksp: torch.Tensor) -> torch.Tensor:
+        """ksp (nc, *trj_size) -> alphas (nsub, *im_size)."""
+        wy = (ksp * self.dcf)[:, None] * self.temporal_funcs.conj()                 # nc nseg *trj
+        pwy = wy[:, None] * self._phi_b().conj()                                     # nc nsub nseg *trj
+        f = self.nufft.adjoint(pwy[None], self.trj[None])[0]                         # nc nsub nseg *im
+        sf = (f * self.mps.conj()[:, None, None]).sum(dim=0)                         # nsub nseg *im
+        return (sf * self.spatial_funcs.conj()).sum(dim=1)
+
+    def normal(self, alphas: torch.Tensor) -> torch.Tensor:
+        if not self.use_toeplitz:
+            return self.adjoint(self.forward(alphas))
+        d = len(self.im_size)
+        K = self.phi.shape[0]
+        im_os = tuple(self.toep_kerns.shape[-d:])
+        padder = PadLast(im_os, self.im_size)
+        dims = tuple(range(-d, 0))
+        sx = self.mps[:, None] * alphas.type(complex_dtype)[None]                   # nc nsub *im
+        F = fft(padder.forward(sx), dim=dims)                                        # nc nsub *os
+        out = torch.zeros((K, *self.im_size), dtype=complex_dtype, device=alphas.device)
+        for i in range(K):
+            m = (self.toep_kerns[i][None] * F).sum(dim=1)                            # nc *os
+            out[i] = (padder.adjoint(ifft(m, dim=dims)) * self.mps.conj()).sum(dim=0)
         return out

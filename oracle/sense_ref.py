@@ -107,3 +107,78 @@ def cg_sense_recon(A, ksp, max_iter=15, lamda_l2=0.0, max_eigen=None, tolerance=
         return AHb
     return conjugate_gradient(lambda x: A.normal(x) / max_eigen, AHb, num_iters=max_iter, lamda_l2=lamda_l2,
                               tolerance=tolerance)
+
+
+class SubspaceRef:
+    """``subspace_linop`` (linops.py:1496-1772) with every batch size 1: k-space of coil c
+    is sum_l T_l * sum_k phi_k(tr) F(S_c B_l alpha_k); the adjoint mirrors it with the
+    dcf; the Toeplitz normal mixes the coefficient images through the K x K PSF
+    kernels of the weights conj(phi_k1) phi_k2 dcf (:1563-1580, 1711-1772)."""
+
+    def __init__(self, trj, mps, phi, dcf=None, nufft=None, use_toeplitz=False, spatial_funcs=None,
+                 temporal_funcs=None):
+        self.nufft = nufft
+        self.cdt = nufft.cdtype
+        self.im_size = tuple(mps.shape[1:])
+        self.trj_size = tuple(trj.shape[:-1])
+        self.trj = trj.to(torch.float32)
+        self.mps = mps.to(self.cdt)
+        self.phi = phi.to(self.cdt)
+        self.dcf = torch.ones(self.trj_size, dtype=torch.float32) if dcf is None else dcf.to(torch.float32)
+        d, nt = len(self.im_size), len(self.trj_size)
+        self.spatial = (torch.ones((1,) * (1 + d), dtype=self.cdt) if spatial_funcs is None
+                        else spatial_funcs.to(self.cdt))
+        self.temporal = (torch.ones((1,) * (1 + nt), dtype=self.cdt) if temporal_funcs is None
+                         else temporal_funcs.to(self.cdt))
+        self.use_toeplitz = use_toeplitz
+        if use_toeplitz:
+            K = phi.shape[0]
+            pp = self.phi.conj()[:, None, :] * self.phi[None]                     # k1 k2 ntr
+            w = pp.reshape(K * K, *([1] * (nt - 1)), pp.shape[-1]) * self.dcf
+            kerns = [nufft.calc_teoplitz_kernels(self.trj[None], w[None, j:j + 1])[0, 0] for j in range(K * K)]
+            self.toep_kerns = torch.stack(kerns, 0).reshape(K, K, *kerns[0].shape)
+
+    def _phi(self, k):
+        return self.phi[k].reshape(*([1] * (len(self.trj_size) - 1)), -1)
+
+    def forward(self, alphas):
+        nc, K, L = self.mps.shape[0], self.phi.shape[0], self.spatial.shape[0]
+        ksp = torch.zeros((nc, *self.trj_size), dtype=self.cdt)
+        for c in range(nc):
+            for l in range(max(L, self.temporal.shape[0])):
+                sl = self.spatial[min(l, L - 1)]
+                tl = self.temporal[min(l, self.temporal.shape[0] - 1)]
+                acc = torch.zeros(self.trj_size, dtype=self.cdt)
+                for k in range(K):
+                    sx = (self.mps[c] * sl * alphas[k])[None, None]
+                    acc += self.nufft.forward(sx, self.trj[None])[0, 0] * self._phi(k)
+                ksp[c] += acc * tl
+        return ksp
+
+    def adjoint(self, ksp):
+        nc, K, L = self.mps.shape[0], self.phi.shape[0], self.spatial.shape[0]
+        out = torch.zeros((K, *self.im_size), dtype=self.cdt)
+        for c in range(nc):
+            y = ksp[c] * self.dcf
+            for l in range(max(L, self.temporal.shape[0])):
+                sl = self.spatial[min(l, L - 1)]
+                wy = y * self.temporal[min(l, self.temporal.shape[0] - 1)].conj()
+                for k in range(K):
+                    f = self.nufft.adjoint((wy * self._phi(k).conj())[None, None], self.trj[None])[0, 0]
+                    out[k] += f * self.mps[c].conj() * sl.conj()
+        return out
+
+    def normal(self, alphas):
+        if not self.use_toeplitz:
+            return self.adjoint(self.forward(alphas))
+        d = len(self.im_size)
+        K = self.phi.shape[0]
+        pad = PadLastRef(self.toep_kerns.shape[-d:], self.im_size)
+        dims = tuple(range(-d, 0))
+        out = torch.zeros((K, *self.im_size), dtype=self.cdt)
+        for c in range(self.mps.shape[0]):
+            F = fft_c(pad.forward((self.mps[c] * alphas.to(self.cdt))[None]), dims)[0]   # K *os
+            for i in range(K):
+                m = (self.toep_kerns[i] * F).sum(0)
+                out[i] += pad.adjoint(ifft_c(m[None], dims))[0] * self.mps[c].conj()
+        return out

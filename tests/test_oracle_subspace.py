@@ -1,0 +1,51 @@
+"""Pins the subspace SENSE oracle (no GPU): with one coefficient and phi = 1 it is the
+SENSE oracle; with several it is the phi-weighted sum of per-coefficient SENSE
+applications (a second code path); adjoint identity; Toeplitz normal close to
+adjoint(forward)."""
+
+import torch
+
+from helpers import cdot, rel_l2
+from mr_recon_amd import synth
+from oracle.nufft_ref import TritonNufftRef
+from oracle.sense_ref import SenseRef, SubspaceRef
+
+
+def _case(K, seed=0):
+    g = torch.Generator().manual_seed(seed)
+    im = (12, 12)
+    trj = ((torch.rand((16, 3, 5, 2), generator=g) - 0.5) * 12).to(torch.float32)   # nro npe ntr d
+    mps = synth.coil_maps(3, im, seed=seed).to(torch.complex128)
+    phi = torch.randn((K, 5), dtype=torch.complex128, generator=g)
+    alphas = torch.randn((K, *im), dtype=torch.complex128, generator=g)
+    return im, trj, mps, phi, alphas
+
+
+def test_subspace_reduces_to_sense_and_sums_per_coefficient():
+    im, trj, mps, phi, alphas = _case(3)
+    nf = TritonNufftRef(im, 2.0, 6, dtype=torch.float64)
+    A = SubspaceRef(trj, mps, phi, nufft=nf)
+    one = SubspaceRef(trj, mps, torch.ones((1, 5), dtype=torch.complex128), nufft=nf)
+    S = SenseRef(trj, mps, nufft=nf)
+    assert rel_l2(one.forward(alphas[:1]), S.forward(alphas[0])) < 1e-12
+    y = A.forward(alphas)
+    ref = sum(S.forward(alphas[k]) * phi[k] for k in range(3))
+    assert rel_l2(y, ref) < 1e-12
+    ksp = torch.randn(y.shape, dtype=torch.complex128, generator=torch.Generator().manual_seed(4))
+    x = A.adjoint(ksp)
+    assert rel_l2(x[1], S.adjoint(ksp * phi[1].conj())) < 1e-12
+    assert abs(cdot(ksp, y) - cdot(x, alphas)) < 1e-10 * abs(cdot(ksp, y))
+
+
+def test_subspace_segments_and_toeplitz():
+    im, trj, mps, phi, alphas = _case(2, seed=1)
+    g = torch.Generator().manual_seed(2)
+    sp = torch.randn((2, *im), dtype=torch.complex128, generator=g)
+    tp = torch.randn((2, *trj.shape[:-1]), dtype=torch.complex128, generator=g)
+    nf = TritonNufftRef(im, 2.0, 6, dtype=torch.float64)
+    A = SubspaceRef(trj, mps, phi, nufft=nf, spatial_funcs=sp, temporal_funcs=tp)
+    y = A.forward(alphas)
+    ksp = torch.randn(y.shape, dtype=torch.complex128, generator=g)
+    assert abs(cdot(ksp, y) - cdot(A.adjoint(ksp), alphas)) < 1e-10 * abs(cdot(ksp, y))
+    T = SubspaceRef(trj, mps, phi, nufft=nf, use_toeplitz=True)
+    assert rel_l2(T.normal(alphas), T.adjoint(T.forward(alphas))) < 1e-3
