@@ -156,6 +156,15 @@ int mra_toeplitz_fused(int ndim, const int* N, const int* G, const int* center, 
                        long long ns, long long s_stride, const void* kern, void* work, int accumulate, void* out,
                        const void* const* tw, void* stream);
 
+/* ---- nearest-grid sampling ------------------------------------------------------------------
+ * Gather / scatter-add at one flat grid index per sample: gridded_nufft's forward_interp_only
+ * and adjoint_grid_only (fourier.py:1250-1294, via indexing.multi_index / multi_grid,
+ * indexing.py:29-79). idx values must lie in [0, gstride); the scatter adds into grid. */
+int mra_nn_gather(const void* grid, long long gstride, const int* idx, long long M, int B, float scale, void* out,
+                  void* stream);
+int mra_nn_scatter(const void* ksp, const int* idx, long long M, int B, float scale, void* grid, long long gstride,
+                   void* stream);
+
 /* ---- density compensation ---------------------------------------------------------------
  * Elementwise steps of the Pipe-Menon iteration behind density_compensation (algs.py:20-106,
  * which calls sigpy.mri.pipe_menon_dcf): w /= |G^H G w| after the gridding and interpolation

@@ -6,6 +6,7 @@ Mirrors the reference ``mr_recon.fourier`` for the hot path:
   ``NUFFT``                     fourier.py:107-352 (base class, ``normal_toeplitz``)
   ``hip_nufft``                 fourier.py:354-582 (``triton_nufft``; alias kept)
   ``sigpy_nufft``               fourier.py:728-1033 (same kernels, sigpy's apodization)
+  ``gridded_nufft``             fourier.py:1213-1326 (nearest-grid gather / scatter)
 
 Transform definition (the reference's): forward
     ksp = interp_KB( FFT( pad_G( apod * img / sqrt(prod N) ) ) ) / w^d
@@ -452,6 +453,107 @@ class sigpy_nufft(hip_nufft):
             errs.append((vals * scale - vals_gt).norm())
         return betas[int(torch.stack(errs).argmin())].item()
 
+class gridded_nufft(NUFFT):
+    """Nearest-grid NUFFT (fourier.py:1213-1326): samples rounded to the nearest point of
+    a ``round(oversamp * N)`` grid, a centred ortho FFT, and a gather (forward) or
+    scatter-add (adjoint) at those points, times oversamp^(d/2).
+
+    ``rescale_trajectory`` turns grid-unit coordinates into the integer grid indices the
+    other methods take (as the reference's callers do). The gather and scatter are the
+    HIP ``mra_nn_gather`` / ``mra_nn_scatter`` kernels at one flat index per sample;
+    the FFTs are torch's (rocFFT) on the centred layout.
+    """
+
+    def __init__(self, im_size: tuple, oversamp: Optional[float] = 1.0):
+        super().__init__(im_size)
+        self.im_size_os = tuple(round(i * oversamp) for i in self.im_size)
+        self.oversamp = oversamp
+
+    def rescale_trajectory(self, trj: torch.Tensor) -> torch.Tensor:
+        """k -> round(k * oversamp + G/2) mod G as int32 (fourier.py:1222-1233)."""
+        trj_rs = trj * self.oversamp
+        for i in range(trj_rs.shape[-1]):
+            n_over_2 = self.im_size_os[i] / 2
+            trj_rs[..., i] = (trj_rs[..., i] + n_over_2).round() % self.im_size_os[i]
+        return trj_rs.type(torch.int32)
+
+    def _flat_index(self, trj_n: torch.Tensor, G) -> torch.Tensor:
+        """Flat index of each sample's grid point; indices follow Python's rule (-G..G-1)."""
+        t = trj_n.reshape(-1, len(G)).to(torch.int64)
+        Gt = torch.tensor(G, dtype=torch.int64, device=t.device)
+        if bool(((t < -Gt) | (t >= Gt)).any()):
+            raise IndexError("gridded_nufft: trajectory index out of range of the grid")
+        t = torch.remainder(t, Gt)
+        idx = torch.zeros(t.shape[0], dtype=torch.int64, device=t.device)
+        for d in range(len(G)):
+            idx = idx * G[d] + t[:, d]
+        return idx.to(torch.int32).contiguous()
+
+    def forward_FT_only(self, img: torch.Tensor) -> torch.Tensor:
+        """centred resize to the oversampled grid -> centred ortho FFT (fourier.py:1235-1248)."""
+        d = len(self.im_size)
+        img_os = resize(img, tuple(img.shape[:-d]) + self.im_size_os)
+        return fft(img_os, dim=tuple(range(-d, 0)))
+
+    def forward_interp_only(self, ksp_os: torch.Tensor, trj: torch.Tensor) -> torch.Tensor:
+        """ksp_os (N, *batch, *G) at integer trj (N, *trj_batch, d) * oversamp^(d/2)."""
+        _lib.require_device(ksp_os, trj)
+        d = trj.shape[-1]
+        G = tuple(ksp_os.shape[-d:])
+        N = trj.shape[0]
+        batch = tuple(ksp_os.shape[1:-d])
+        B = int(math.prod(batch))
+        M = int(math.prod(trj.shape[1:-1]))
+        out = torch.empty((N, B, M), dtype=complex_dtype, device=ksp_os.device)
+        scale = float(self.oversamp ** (d / 2))
+        for i in range(N):
+            g = ksp_os[i].to(complex_dtype).contiguous()
+            idx = self._flat_index(trj[i], G)
+            _lib.check(_lib.lib().mra_nn_gather(_lib.ptr(g), int(math.prod(G)), _lib.ptr(idx), M, B, scale,
+                                                _lib.ptr(out[i]), _lib.stream_ptr(g.device)))
+        return out.reshape(N, *batch, *trj.shape[1:-1])
+
+    def adjoint_iFT_only(self, ksp_os: torch.Tensor) -> torch.Tensor:
+        """centred ortho iFFT -> centred crop (fourier.py:1267-1276)."""
+        d = len(self.im_size)
+        img_os = ifft(ksp_os, dim=tuple(range(-d, 0)))
+        return resize(img_os, tuple(img_os.shape[:-d]) + self.im_size)
+
+    def adjoint_grid_only(self, ksp: torch.Tensor, trj: torch.Tensor) -> torch.Tensor:
+        """ksp (N, *batch, *trj_batch) added into the oversampled grid at integer trj."""
+        _lib.require_device(ksp, trj)
+        d = trj.shape[-1]
+        G = self.im_size_os
+        N = trj.shape[0]
+        nt = trj.ndim - 2
+        batch = tuple(ksp.shape[1:ksp.ndim - nt])
+        B = int(math.prod(batch))
+        M = int(math.prod(trj.shape[1:-1]))
+        out = torch.zeros((N, B, *G), dtype=complex_dtype, device=ksp.device)
+        scale = float(self.oversamp ** (d / 2))
+        for i in range(N):
+            y = ksp[i].reshape(B, M).to(complex_dtype).contiguous()
+            idx = self._flat_index(trj[i], G)
+            _lib.check(_lib.lib().mra_nn_scatter(_lib.ptr(y), _lib.ptr(idx), M, B, scale, _lib.ptr(out[i]),
+                                                 int(math.prod(G)), _lib.stream_ptr(y.device)))
+        return out.reshape(N, *batch, *G)
+
+    def forward(self, img: torch.Tensor, trj: torch.Tensor) -> torch.Tensor:
+        return self.forward_interp_only(self.forward_FT_only(img), trj)
+
+    def adjoint(self, ksp: torch.Tensor, trj: torch.Tensor) -> torch.Tensor:
+        return self.adjoint_iFT_only(self.adjoint_grid_only(ksp, trj))
+
+    def calc_teoplitz_kernels(self, trj: torch.Tensor, weights: Optional[torch.Tensor] = None,
+                              os_factor: Optional[float] = None):
+        """PSF kernels on the oversampled grid (fourier.py:1296-1326; os_factor unused)."""
+        os_factor = self.oversamp
+        im_size_os = tuple(round(i * os_factor) for i in self.im_size)
+        nufft_os = gridded_nufft(im_size_os, oversamp=1.0)
+        return calc_toep_kernel_helper(nufft_os.adjoint, trj.type(torch.int32), weights) * (
+            os_factor ** len(self.im_size))
+
+
 __all__ = ["fft", "ifft", "calc_toep_kernel_helper", "NUFFT", "hip_nufft", "triton_nufft", "kb_beta",
-           "sigpy_nufft", "sigpy_beta",
+           "sigpy_nufft", "sigpy_beta", "gridded_nufft",
            "toeplitz_unshifted", "batch_iterator"]

@@ -228,6 +228,51 @@ class SigpyNufftRef(TritonNufftRef):
             os_factor ** len(self.im_size))
 
 
+class GriddedNufftRef:
+    """fourier.py:1213-1326 on CPU: rescale to round(k s + G/2) mod G, centred resize and
+    ortho FFT, then torch advanced indexing (multi_index, indexing.py:29-51) or
+    index_add_ (multi_grid, indexing.py:53-79), times s^(d/2)."""
+
+    def __init__(self, im_size, oversamp=1.0, dtype=torch.float32):
+        self.im_size = tuple(im_size)
+        self.im_size_os = tuple(round(i * oversamp) for i in self.im_size)
+        self.oversamp = oversamp
+        self.cdtype = torch.complex64 if dtype == torch.float32 else torch.complex128
+
+    def rescale_trajectory(self, trj):
+        t = trj * self.oversamp
+        for i in range(t.shape[-1]):
+            t[..., i] = (t[..., i] + self.im_size_os[i] / 2).round() % self.im_size_os[i]
+        return t.type(torch.int32)
+
+    def forward(self, img, trj):
+        d = len(self.im_size)
+        g = fft_c(resize(img.to(self.cdtype), tuple(img.shape[:-d]) + self.im_size_os), tuple(range(-d, 0)))
+        outs = []
+        for i in range(trj.shape[0]):
+            t = trj[i].long()
+            outs.append(g[i][(Ellipsis,) + tuple(t[..., j] for j in range(d))])
+        return torch.stack(outs, 0) * self.oversamp ** (d / 2)
+
+    def adjoint(self, ksp, trj):
+        d = trj.shape[-1]
+        G = self.im_size_os
+        nt = trj.ndim - 2
+        out = torch.zeros((trj.shape[0], *ksp.shape[1:ksp.ndim - nt], *G), dtype=self.cdtype)
+        for i in range(trj.shape[0]):
+            t = trj[i].long().reshape(-1, d)
+            flat = torch.zeros(t.shape[0], dtype=torch.int64)
+            for j in range(d):
+                flat = flat * G[j] + t[:, j]
+            y = ksp[i].to(self.cdtype).reshape(-1, t.shape[0])
+            o = torch.zeros((y.shape[0], math.prod(G)), dtype=self.cdtype)
+            o.index_add_(1, flat, y)
+            out[i] = o.reshape(out.shape[1:])
+        g = out * self.oversamp ** (d / 2)
+        img = ifft_c(g, tuple(range(-d, 0)))
+        return resize(img, tuple(img.shape[:-d]) + self.im_size)
+
+
 def calc_toep_kernel_helper(nufft_adj_os, trj, weights=None, cdtype=torch.complex64):
     """fourier.py:63-105."""
     if weights is None:

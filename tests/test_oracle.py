@@ -11,7 +11,7 @@ from helpers import cdot, edge_coords, rel_l2
 from mr_recon_amd import synth
 from oracle import bins as obins
 from oracle import interp_ref, kb, ndft, sigpy_ref
-from oracle.nufft_ref import SigpyNufftRef, TritonNufftRef, fft_c, resize
+from oracle.nufft_ref import GriddedNufftRef, SigpyNufftRef, TritonNufftRef, fft_c, resize
 from oracle.sense_ref import SenseRef, conjugate_gradient
 
 
@@ -299,3 +299,20 @@ def test_sigpy_interp_vs_bruteforce(D, G, w):
     assert abs(cdot(y, got) - cdot(g, vals)) < 1e-10 * abs(cdot(y, got))
     if w >= 2:  # the reference's torch path masks |u| <= w/2 as well, which only bites below 2
         assert rel_l2(got, interp_ref.ungrid(vals, k, w, beta=2.5, dtype=torch.float64)) < 1e-6
+
+
+@pytest.mark.parametrize("os_", [1.0, 2.0])
+def test_gridded_nufft_oracle_is_exact_on_its_lattice(os_):
+    """On k in (1/os) Z^d the nearest-grid NUFFT is the exact DFT: == the FP64 NDFT;
+    its adjoint is the exact adjoint."""
+    im = (12, 10)
+    rng = np.random.default_rng(int(os_))
+    k = (np.round((rng.random((300, 2)) - 0.5) * np.array(im) * os_) / os_).astype(np.float32)
+    x = synth.random_image(im, seed=2).to(torch.complex128)
+    nf = GriddedNufftRef(im, os_, dtype=torch.float64)
+    t = nf.rescale_trajectory(torch.from_numpy(k))[None]
+    y = nf.forward(x[None], t)[0]
+    assert rel_l2(y, torch.from_numpy(ndft.ndft_forward(x.numpy(), k, im))) < 1e-12
+    yy = synth.random_image((300,), seed=3).to(torch.complex128)
+    z = nf.adjoint(yy[None], t)[0]
+    assert rel_l2(z, torch.from_numpy(ndft.ndft_adjoint(yy.numpy(), k, im))) < 1e-12
