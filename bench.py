@@ -6,8 +6,9 @@ gradients (image and trajectory) through a 2D SENSE NUFFT plus an unrolled CG-SE
 against the plain per-coil differentiable path; ``cg2d`` -- 2D spiral CG-SENSE, 30
 iterations, Toeplitz normal operator; ``cg3d`` -- the 3D kooshball above, 15 CG-SENSE
 iterations (Toeplitz and NUFFT normal operators); ``dcf`` -- Pipe-Menon density
-compensation of the kooshball trajectory below, 30 iterations. Each prints its own
-JSON line.
+compensation of the kooshball trajectory below, 30 iterations; ``rows`` -- sigpy_nufft,
+gridded_nufft, subspace_linop and type3_nufft forward+adjoint. Each prints its own
+JSON line(s).
 
 Workload (the headline configuration): 128^3 complex64 image, 32 smooth coil
 maps, 20,000 Fibonacci-sphere spokes x 256 readout points (5.12 M samples per
@@ -419,13 +420,129 @@ def run_dcf(args):
     print(json.dumps(rec), flush=True)
 
 
+def run_rows(args):
+    """One JSON line per widened operator, each timed as forward + adjoint on the GPU beside
+    its oracle on a bounded CPU sample (scaled linearly by samples):
+    sigpy_nufft (w 6, sigma 1.25) and gridded_nufft (sigma 2) on the 3D kooshball with 32
+    coils; subspace_linop 2D 256^2 x 16 coils x 5 coefficients on 32 spiral interleaves x
+    1024 x 8 time points; type3_nufft 2D 128^2 at 64 x 1024 samples (w 4, sigma 2)."""
+    import numpy as np
+
+    from mr_recon_amd import synth
+    from mr_recon_amd.fourier import gridded_nufft, sigpy_nufft
+    from mr_recon_amd.linops import subspace_linop, type3_nufft
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator().manual_seed(0)
+
+    def timed_pair(fwd, adj, x):
+        for _ in range(args.warmup):
+            adj(fwd(x))
+        t, _ = _timed(lambda: adj(fwd(x)), args.steps)
+        return t
+
+    def emit(name, unit_cfg, t, cpu_s, cpu_sample, dtype="c64"):
+        rec = {"metric": f"{name} fwd+adj/s", "value": 1.0 / t, "unit": "fwd+adj/s", "n_gpus": 1,
+               "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3, "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+               "config": dict(workload=name, **unit_cfg)}
+        if cpu_s is not None:
+            rec["cpu_baseline"] = {"value": 1.0 / cpu_s, "unit": "fwd+adj/s", "cores": torch.get_num_threads(),
+                                   "kind": "port", "sample": cpu_sample}
+        print(json.dumps(rec), flush=True)
+
+    def cpu_time(fn):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+
+    cpu = not args.no_cpu_baseline
+    trj = synth.kooshball_3d(NSPOKES, NREAD, N)
+    M = trj.shape[0] * trj.shape[1]
+    img = synth.random_image((1, C, N, N, N), seed=1).to(dev)
+    t3 = trj.reshape(1, -1, 3).to(dev)
+    nsub_sp = 100  # spokes in the CPU sample (one coil)
+
+    # sigpy_nufft, sigpy's defaults (w 6, sigma 1.25)
+    sn = sigpy_nufft((N,) * 3)
+    t = timed_pair(lambda x: sn.forward(x, t3), lambda y: sn.adjoint(y, t3), img)
+    cs = None
+    if cpu:
+        from oracle.nufft_ref import SigpyNufftRef
+
+        sub = synth.kooshball_3d(nsub_sp, NREAD, N).reshape(1, -1, 3)
+        ref = SigpyNufftRef((N,) * 3)
+        x1 = img[:, :1].cpu()
+        t_all = cpu_time(lambda: ref.adjoint(ref.forward(x1, sub), sub))
+        t_fft = cpu_time(lambda: ref.adjoint_iFT_only(ref.forward_FT_only(x1)))
+        cs = (t_fft + max(t_all - t_fft, 0.0) * (M / sub.shape[1])) * C
+    emit("sigpy_nufft 3D 128^3 x 32 coils, kooshball 5.12 M samples", {"width": 6, "oversamp": 1.25}, t, cs,
+         f"1 coil through the oracle: FFT part as measured, sample part on {nsub_sp} spokes scaled to "
+         f"all samples; x{C} coils")
+    del sn
+
+    # gridded_nufft (sigma 2)
+    gn = gridded_nufft((N,) * 3, 2.0)
+    ti = gn.rescale_trajectory(t3.clone())
+    t = timed_pair(lambda x: gn.forward(x, ti), lambda y: gn.adjoint(y, ti), img)
+    cs = None
+    if cpu:
+        from oracle.nufft_ref import GriddedNufftRef
+
+        ref = GriddedNufftRef((N,) * 3, 2.0)
+        tic = ti.cpu()
+        cs = cpu_time(lambda: ref.adjoint(ref.forward(x1, tic), tic)) * C
+    emit("gridded_nufft 3D 128^3 x 32 coils, kooshball 5.12 M samples", {"oversamp": 2.0}, t, cs,
+         f"1 coil, all samples through the oracle, scaled x{C} coils")
+    del gn, img, t3
+
+    # subspace_linop
+    n2, c2, K, ntr = 256, 16, 5, 8
+    sp = synth.spiral_2d(32, 1024, n2).permute(1, 0, 2)[:, :, None, :].expand(1024, 32, ntr, 2)
+    sp = (sp * (1 - 0.01 * torch.arange(ntr).reshape(1, 1, ntr, 1))).contiguous()  # distinct time points
+    mps = synth.coil_maps(c2, (n2, n2), seed=2)
+    phi = torch.randn((K, ntr), dtype=torch.complex64, generator=g)
+    al = torch.randn((K, n2, n2), dtype=torch.complex64, generator=g)
+    A = subspace_linop((n2, n2), sp.to(dev), mps.to(dev), phi.to(dev))
+    t = timed_pair(A.forward, A.adjoint, al.to(dev))
+    cs = None
+    if cpu:
+        from oracle.nufft_ref import TritonNufftRef
+        from oracle.sense_ref import SubspaceRef
+
+        spc = sp[:, :8].contiguous()
+        R = SubspaceRef(spc, mps[:2], phi, nufft=TritonNufftRef((n2, n2)))
+        cs = cpu_time(lambda: R.adjoint(R.forward(al))) * (c2 / 2) * 4
+    emit("subspace_linop 2D 256^2 x 16 coils x 5 coefficients, 32x1024x8 samples",
+         {"oversamp": 1.25, "width": 6}, t, cs,
+         "2 coils, 8 of 32 interleaves through the oracle, scaled x8 coils and x4 interleaves")
+    del A
+
+    # type3_nufft
+    n3 = 128
+    axes = torch.meshgrid(torch.arange(n3) - n3 / 2, torch.arange(n3) - n3 / 2, indexing="ij")
+    phis = torch.stack(axes, 0) / n3 + 0.02 * torch.randn((2, n3, n3), generator=g)
+    alphas = (torch.rand((2, 64 * 1024), generator=g) - 0.5) * n3
+    x3 = torch.randn((1, n3, n3), dtype=torch.complex64, generator=g)
+    T3 = type3_nufft(phis.to(dev), alphas.to(dev), 2.0, 4.0)
+    t = timed_pair(T3.forward, T3.adjoint, x3.to(dev))
+    cs = None
+    if cpu:
+        from oracle.type3_ref import Type3Ref
+
+        R3 = Type3Ref(phis, alphas, 2.0, 4.0)
+        cs = cpu_time(lambda: R3.adjoint(R3.forward(x3)))
+    emit("type3_nufft 2D 128^2, 65536 samples", {"width": 4.0, "oversamp": 2.0}, t, cs,
+         "the whole workload through the oracle")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="nufft3d", choices=["nufft3d", "batched", "grad", "cg2d", "cg3d", "dcf"])
+    ap.add_argument("--workload", default="nufft3d", choices=["nufft3d", "batched", "grad", "cg2d", "cg3d", "dcf", "rows"])
     ap.add_argument("--samples-per-item", default="auto",
                     help="batched workload: work-item size (an int, or 'auto' for the autotuner)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -446,6 +563,8 @@ def main():
         return run_cg(args, args.workload == "cg3d")
     if args.workload == "dcf":
         return run_dcf(args)
+    if args.workload == "rows":
+        return run_rows(args)
 
     from mr_recon_amd.fourier import hip_nufft
     from mr_recon_amd.linops import sense_linop

@@ -460,8 +460,11 @@ class gridded_nufft(NUFFT):
 
     ``rescale_trajectory`` turns grid-unit coordinates into the integer grid indices the
     other methods take (as the reference's callers do). The gather and scatter are the
-    HIP ``mra_nn_gather`` / ``mra_nn_scatter`` kernels at one flat index per sample;
-    the FFTs are torch's (rocFFT) on the centred layout.
+    HIP ``mra_nn_gather`` / ``mra_nn_scatter`` kernels at one flat index per sample.
+    ``forward`` / ``adjoint`` run the fused pad+FFT and iFFT+crop kernels in the unshifted
+    layout (the index carries the centring); the split methods keep the reference's
+    centred intermediate grid and use torch's FFT, as does any grid the fused
+    transforms do not cover.
     """
 
     def __init__(self, im_size: tuple, oversamp: Optional[float] = 1.0):
@@ -477,13 +480,15 @@ class gridded_nufft(NUFFT):
             trj_rs[..., i] = (trj_rs[..., i] + n_over_2).round() % self.im_size_os[i]
         return trj_rs.type(torch.int32)
 
-    def _flat_index(self, trj_n: torch.Tensor, G) -> torch.Tensor:
-        """Flat index of each sample's grid point; indices follow Python's rule (-G..G-1)."""
+    def _flat_index(self, trj_n: torch.Tensor, G, unshifted: bool = False) -> torch.Tensor:
+        """Flat index of each sample's grid point; indices follow Python's rule (-G..G-1).
+        ``unshifted``: position in the unshifted FFT layout, where centred index i sits
+        at (i - G//2) mod G."""
         t = trj_n.reshape(-1, len(G)).to(torch.int64)
         Gt = torch.tensor(G, dtype=torch.int64, device=t.device)
         if bool(((t < -Gt) | (t >= Gt)).any()):
             raise IndexError("gridded_nufft: trajectory index out of range of the grid")
-        t = torch.remainder(t, Gt)
+        t = torch.remainder(t - Gt // 2 if unshifted else t, Gt)
         idx = torch.zeros(t.shape[0], dtype=torch.int64, device=t.device)
         for d in range(len(G)):
             idx = idx * G[d] + t[:, d]
@@ -538,11 +543,53 @@ class gridded_nufft(NUFFT):
                                                  int(math.prod(G)), _lib.stream_ptr(y.device)))
         return out.reshape(N, *batch, *G)
 
+    def _fused(self) -> bool:
+        return all(g >= n for g, n in zip(self.im_size_os, self.im_size)) and ops.fused_supported(self.im_size_os)
+
     def forward(self, img: torch.Tensor, trj: torch.Tensor) -> torch.Tensor:
-        return self.forward_interp_only(self.forward_FT_only(img), trj)
+        """forward_interp_only(forward_FT_only(img)); fused: one pad+FFT kernel chain writes the
+        unshifted spectrum and the gather reads it at the shifted index."""
+        if not self._fused():
+            return self.forward_interp_only(self.forward_FT_only(img), trj)
+        _lib.require_device(img, trj)
+        d = len(self.im_size)
+        G = self.im_size_os
+        N = trj.shape[0]
+        batch = tuple(img.shape[1:-d])
+        B = int(math.prod(batch))
+        M = int(math.prod(trj.shape[1:-1]))
+        out = torch.empty((N, B, M), dtype=complex_dtype, device=img.device)
+        for n in range(N):
+            x = img[n if img.shape[0] > 1 else 0].reshape(B, *self.im_size).to(complex_dtype)
+            g = ops.pad_fft(x, self.im_size, G, scale=1.0 / math.sqrt(math.prod(G)))
+            idx = self._flat_index(trj[n], G, unshifted=True)
+            _lib.check(_lib.lib().mra_nn_gather(_lib.ptr(g), int(math.prod(G)), _lib.ptr(idx), M, B,
+                                                float(self.oversamp ** (d / 2)), _lib.ptr(out[n]),
+                                                _lib.stream_ptr(g.device)))
+        return out.reshape(N, *batch, *trj.shape[1:-1])
 
     def adjoint(self, ksp: torch.Tensor, trj: torch.Tensor) -> torch.Tensor:
-        return self.adjoint_iFT_only(self.adjoint_grid_only(ksp, trj))
+        """adjoint_iFT_only(adjoint_grid_only(ksp)); fused: scatter into the unshifted layout,
+        then one iFFT+crop kernel chain."""
+        if not self._fused():
+            return self.adjoint_iFT_only(self.adjoint_grid_only(ksp, trj))
+        _lib.require_device(ksp, trj)
+        d = len(self.im_size)
+        G = self.im_size_os
+        N = trj.shape[0]
+        nt = trj.ndim - 2
+        batch = tuple(ksp.shape[1:ksp.ndim - nt])
+        B = int(math.prod(batch))
+        M = int(math.prod(trj.shape[1:-1]))
+        out = torch.empty((N, B, *self.im_size), dtype=complex_dtype, device=ksp.device)
+        for n in range(N):
+            y = ksp[n].reshape(B, M).to(complex_dtype).contiguous()
+            idx = self._flat_index(trj[n], G, unshifted=True)
+            g = torch.zeros((B, *G), dtype=complex_dtype, device=ksp.device)
+            _lib.check(_lib.lib().mra_nn_scatter(_lib.ptr(y), _lib.ptr(idx), M, B, float(self.oversamp ** (d / 2)),
+                                                 _lib.ptr(g), int(math.prod(G)), _lib.stream_ptr(y.device)))
+            ops.ifft_crop(g, self.im_size, G, B, scale=1.0 / math.sqrt(math.prod(G)), out=out[n])
+        return out.reshape(N, *batch, *self.im_size)
 
     def calc_teoplitz_kernels(self, trj: torch.Tensor, weights: Optional[torch.Tensor] = None,
                               os_factor: Optional[float] = None):

@@ -34,6 +34,11 @@ def test_gridded_vs_oracle(dev, im, os_):
     assert z.shape == (2, 3, *im)
     assert rel_l2(z, rn.adjoint(yy, tr)) < TOL
     assert abs(cdot(yy, y) - cdot(z, x)) < 1e-5 * abs(cdot(yy, y))
+    # the split methods (centred intermediate grid) agree with the fused transforms
+    ys = hn.forward_interp_only(hn.forward_FT_only(x.to(dev)), t)
+    assert rel_l2(ys, y) < TOL
+    zs = hn.adjoint_iFT_only(hn.adjoint_grid_only(yy.to(dev), t))
+    assert rel_l2(zs, z) < TOL
 
 
 def test_gridded_exact_on_lattice_and_toeplitz(dev):
