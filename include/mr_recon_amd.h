@@ -164,6 +164,12 @@ int mra_nn_gather(const void* grid, long long gstride, const int* idx, long long
                   void* stream);
 int mra_nn_scatter(const void* ksp, const int* idx, long long M, int B, float scale, void* grid, long long gstride,
                    void* stream);
+/* The same sums over a stable sort by flat index (perm: sorted -> sample), cut into chunks of
+ * one grid point's samples (chunk c = sorted positions cstart[c] .. cend[c]-1 at point cidx[c]):
+ * each chunk is summed in sorted order and added with one atomic per value. */
+int mra_nn_scatter_chunks(const void* ksp, const int* perm, const int* cstart, const int* cend, const int* cidx,
+                          long long NC, long long M, int B, float scale, void* grid, long long gstride,
+                          void* stream);
 
 /* ---- density compensation ---------------------------------------------------------------
  * Elementwise steps of the Pipe-Menon iteration behind density_compensation (algs.py:20-106,

@@ -78,6 +78,7 @@ SIGNATURES = {
     "mra_cg_step_seg": (_i, [_vp, _vp, _vp, _vp, _ll, _i, _f, _vp, _vp, _vp]),
     "mra_nn_gather": (_i, [_vp, _ll, _vp, _ll, _i, _f, _vp, _vp]),
     "mra_nn_scatter": (_i, [_vp, _vp, _ll, _i, _f, _vp, _ll, _vp]),
+    "mra_nn_scatter_chunks": (_i, [_vp, _vp, _vp, _vp, _vp, _ll, _ll, _i, _f, _vp, _ll, _vp]),
     "mra_dcf_partials": (_i, []),
     "mra_dcf_update": (_i, [_vp, _vp, _ll, _vp]),
     "mra_dcf_normalize": (_i, [_vp, _vp, _vp, _ll, _vp, _vp]),

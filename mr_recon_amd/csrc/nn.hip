@@ -49,6 +49,38 @@ __global__ __launch_bounds__(kThreads) void scatter_kernel(const float2* __restr
   }
 }
 
+// Segmented form of the scatter: the samples sorted by grid index (perm: sorted ->
+// sample) and cut into chunks of at most one run's samples (a run = all samples of one
+// grid point), chunk c = sorted positions cstart[c] .. cend[c]-1 at point cidx[c]. One
+// thread per (chunk, batch entry) sums its chunk in sorted order and adds the sum with
+// one atomic: a point hit by many samples (the centre of a radial trajectory) takes a
+// few chunk sums instead of one atomic per sample.
+__global__ __launch_bounds__(kThreads) void scatter_chunks_kernel(const float2* __restrict__ ksp,
+                                                                  const int* __restrict__ perm,
+                                                                  const int* __restrict__ cstart,
+                                                                  const int* __restrict__ cend,
+                                                                  const int* __restrict__ cidx, long long NC,
+                                                                  long long M, int B, float scale,
+                                                                  float2* __restrict__ grid, long long gstride) {
+  const long long n = NC * B;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    const long long c = t / B;
+    const int b = (int)(t - c * B);
+    const float2* kb = ksp + b * M;
+    float sx = 0.f, sy = 0.f;
+    const int j1 = cend[c];
+#pragma unroll 4
+    for (int j = cstart[c]; j < j1; ++j) {
+      const float2 v = kb[perm[j]];
+      sx += v.x;
+      sy += v.y;
+    }
+    float* g = reinterpret_cast<float*>(grid + b * gstride + cidx[c]);
+    atomicAdd(g, sx * scale);
+    atomicAdd(g + 1, sy * scale);
+  }
+}
+
 }  // namespace nn
 }  // namespace mra
 
@@ -77,6 +109,21 @@ int mra_nn_scatter(const void* ksp, const int* idx, long long M, int B, float sc
   nn::scatter_kernel<<<nn::blocks_for(M), nn::kThreads, 0, (hipStream_t)stream>>>(
       (const float2*)ksp, idx, M, B, scale, (float2*)grid, gstride);
   MRA_CHECK_LAUNCH("nn_scatter");
+  MRA_TRY_END
+}
+
+// Adds scale * (sum of chunk c's samples) into grid[b, cidx[c]] (the caller zeroes the
+// grid). perm / cstart / cend / cidx come from sorting the samples' flat indices and
+// cutting each run of equal indices into chunks.
+int mra_nn_scatter_chunks(const void* ksp, const int* perm, const int* cstart, const int* cend, const int* cidx,
+                          long long NC, long long M, int B, float scale, void* grid, long long gstride,
+                          void* stream) {
+  MRA_TRY_BEGIN
+  if (NC <= 0 || M <= 0 || B <= 0 || gstride <= 0) MRA_FAIL("nn_scatter_chunks: sizes must be positive");
+  if (!ksp || !perm || !cstart || !cend || !cidx || !grid) MRA_FAIL("nn_scatter_chunks: null pointer");
+  nn::scatter_chunks_kernel<<<nn::blocks_for(NC * B), nn::kThreads, 0, (hipStream_t)stream>>>(
+      (const float2*)ksp, perm, cstart, cend, cidx, NC, M, B, scale, (float2*)grid, gstride);
+  MRA_CHECK_LAUNCH("nn_scatter_chunks");
   MRA_TRY_END
 }
 

@@ -462,9 +462,11 @@ class gridded_nufft(NUFFT):
     other methods take (as the reference's callers do). The gather and scatter are the
     HIP ``mra_nn_gather`` / ``mra_nn_scatter`` kernels at one flat index per sample.
     ``forward`` / ``adjoint`` run the fused pad+FFT and iFFT+crop kernels in the unshifted
-    layout (the index carries the centring); the split methods keep the reference's
-    centred intermediate grid and use torch's FFT, as does any grid the fused
-    transforms do not cover.
+    layout (the index carries the centring), and the adjoint sums each grid point's samples
+    in chunks of at most 64 over a stable sort by index, one atomic per chunk sum
+    (``mra_nn_scatter_chunks``);
+    the split methods keep the reference's centred intermediate grid, torch's FFT and
+    the atomic scatter, as does any grid the fused transforms do not cover.
     """
 
     def __init__(self, im_size: tuple, oversamp: Optional[float] = 1.0):
@@ -493,6 +495,27 @@ class gridded_nufft(NUFFT):
         for d in range(len(G)):
             idx = idx * G[d] + t[:, d]
         return idx.to(torch.int32).contiguous()
+
+    CHUNK = 64  # samples per scatter thread at most
+
+    @staticmethod
+    def _chunks(idx: torch.Tensor, chunk: int = CHUNK):
+        """Samples sorted by flat index (stable), each run of one index cut into chunks of at
+        most ``chunk``: the sort permutation and per chunk its first / past-last sorted
+        position and grid index."""
+        sidx, perm = torch.sort(idx, stable=True)
+        head = torch.ones_like(sidx, dtype=torch.bool)
+        head[1:] = sidx[1:] != sidx[:-1]
+        starts = torch.nonzero(head).reshape(-1)
+        ends = torch.cat([starts[1:], starts.new_tensor([idx.numel()])])
+        nch = (ends - starts + chunk - 1) // chunk
+        run = torch.repeat_interleave(torch.arange(starts.numel(), device=idx.device), nch)
+        first = torch.cumsum(nch, 0) - nch
+        k = torch.arange(run.numel(), device=idx.device) - first[run]
+        cstart = starts[run] + k * chunk
+        cend = torch.minimum(cstart + chunk, ends[run])
+        return (perm.to(torch.int32).contiguous(), cstart.to(torch.int32).contiguous(),
+                cend.to(torch.int32).contiguous(), sidx[starts][run].contiguous())
 
     def forward_FT_only(self, img: torch.Tensor) -> torch.Tensor:
         """centred resize to the oversampled grid -> centred ortho FFT (fourier.py:1235-1248)."""
@@ -584,10 +607,12 @@ class gridded_nufft(NUFFT):
         out = torch.empty((N, B, *self.im_size), dtype=complex_dtype, device=ksp.device)
         for n in range(N):
             y = ksp[n].reshape(B, M).to(complex_dtype).contiguous()
-            idx = self._flat_index(trj[n], G, unshifted=True)
+            perm, cstart, cend, cidx = self._chunks(self._flat_index(trj[n], G, unshifted=True))
             g = torch.zeros((B, *G), dtype=complex_dtype, device=ksp.device)
-            _lib.check(_lib.lib().mra_nn_scatter(_lib.ptr(y), _lib.ptr(idx), M, B, float(self.oversamp ** (d / 2)),
-                                                 _lib.ptr(g), int(math.prod(G)), _lib.stream_ptr(y.device)))
+            _lib.check(_lib.lib().mra_nn_scatter_chunks(_lib.ptr(y), _lib.ptr(perm), _lib.ptr(cstart),
+                                                        _lib.ptr(cend), _lib.ptr(cidx), cidx.numel(), M, B,
+                                                        float(self.oversamp ** (d / 2)), _lib.ptr(g),
+                                                        int(math.prod(G)), _lib.stream_ptr(y.device)))
             ops.ifft_crop(g, self.im_size, G, B, scale=1.0 / math.sqrt(math.prod(G)), out=out[n])
         return out.reshape(N, *batch, *self.im_size)
 
