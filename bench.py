@@ -436,6 +436,10 @@ def run_rows(args):
     g = torch.Generator().manual_seed(0)
 
     def timed_pair(fwd, adj, x):
+        # each row starts from an empty allocator cache: the multi-GB grids of the rows
+        # before it would otherwise leave it fragmented for the small ones after
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
         for _ in range(args.warmup):
             adj(fwd(x))
         t, _ = _timed(lambda: adj(fwd(x)), args.steps)

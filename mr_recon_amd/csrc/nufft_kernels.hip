@@ -1365,6 +1365,9 @@ long long mra_subgrid_points(int ndim, int P) { return subgrid_points(ndim, P); 
 int mra_max_coil_chunk(int ndim, int P) {
   const int n = subgrid_points(ndim, P);
   int c = kLdsBudget / (n * (int)sizeof(float2));
+  // A budget that holds one subgrid still takes two when they fit in 48 KB, so that
+  // large kernels (3D, P = 7: 21.4 KB per coil) read coil pairs.
+  if (c == 1 && 2 * n * (int)sizeof(float2) <= 48 * 1024) c = 2;
   return c < 1 ? 1 : c;
 }
 
