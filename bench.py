@@ -14,9 +14,13 @@ Workload (the headline configuration): 128^3 complex64 image, 32 smooth coil
 maps, 20,000 Fibonacci-sphere spokes x 256 readout points (5.12 M samples per
 coil), Kaiser-Bessel width 4, oversampling 1.25 (grid 160^3). One step = one
 SENSE forward (32 coils, image -> k-space) plus one adjoint (k-space -> image),
-inputs resident in HBM. Prints one JSON line.
+inputs resident in HBM. The same JSON line also carries the other two halves of
+the metric, each against the plain per-problem path on the same GPU: ``batched``
+(the batched workload above, 8 timed replays) and ``grad`` (the gradient workload,
+8 timed evaluations), plus the per-trajectory plan build time (``plan_build_ms``,
+outside the timed steps) and the work-item count of the plan.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--headline-only]
 
 ``roofline`` is reported for the dominant kernel of the step, timed live with
 HIP events around its launches inside the timed loop; its algorithmic bytes
@@ -90,10 +94,32 @@ def step_bytes(M):
     return 16 * N ** 3 + 24 * C * N ** 3 + 56 * C * G ** 3 + 16 * C * M + 8 * 3 * M
 
 
+def cpu_threads():
+    """Threads for the CPU baseline: the host's CPUs, capped at the share this process is
+    given (OMP_NUM_THREADS, when set)."""
+    n = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS")
+    return min(n, int(share)) if share and share.isdigit() and int(share) > 0 else n
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(frac_spokes=1 / 4):
     """Reference CPU path on 1 coil and a spoke subset; scaled to 32 coils and all spokes."""
     from mr_recon_amd import synth
     from oracle.nufft_ref import TritonNufftRef
+
+    torch.set_num_threads(cpu_threads())
 
     nsp = int(NSPOKES * frac_spokes)
     trj = synth.kooshball_3d(nsp, NREAD, N).reshape(1, -1, 3)
@@ -119,6 +145,8 @@ def cpu_baseline(frac_spokes=1 / 4):
         "unit": "NUFFT fwd+adj/s",
         "cores": torch.get_num_threads(),
         "kind": "port",
+        "host": {"cpu_model": cpu_model(), "host_cpus": os.cpu_count(), "threads": torch.get_num_threads(),
+                 "torch": torch.__version__},
         "sample": (f"1 of {C} coils; FFT/pad/crop side on the full 160^3 grid ({t_fft:.2f}s) + interp/grid on "
                    f"{nsp} of {NSPOKES} spokes ({t_samp:.2f}s, 2^18-sample chunks); scaled linearly to {C} coils "
                    f"and all spokes = {per_step:.1f}s per fwd+adj"),
@@ -135,6 +163,10 @@ def _timed(fn, reps):
 
 
 def run_batched(args):
+    print(json.dumps(measure_batched(args.steps, args.warmup, args.samples_per_item)), flush=True)
+
+
+def measure_batched(steps, warmup, samples_per_item="auto"):
     """64 uneven 2D problems: N in {128,192,256}, C in {8,16,32}, radial or spiral, 20k..130k
     samples per coil; 10 CG-SENSE iterations each (lamda 1e-3, max_eigen 1)."""
     import numpy as np
@@ -163,7 +195,7 @@ def run_batched(args):
         spec.append((N, C, trj.shape[0]))
     kw = dict(num_iters=10, lamda_l2=1e-3, max_eigen=1.0)
     t0 = time.time()
-    B = BatchedSense(probs, oversamp=1.25, width=4, samples_per_item=args.samples_per_item)
+    B = BatchedSense(probs, oversamp=1.25, width=4, samples_per_item=samples_per_item)
     torch.cuda.synchronize()
     log(f"batched operators built in {time.time() - t0:.1f}s ({len(B.groups)} groups)")
     # the plain path gets the same (autotuned) work-item size per problem, so both paths
@@ -188,32 +220,41 @@ def run_batched(args):
     B.capture(**kw)
     torch.cuda.synchronize()
     log(f"graph captured in {time.time() - t0:.1f}s")
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         plain()
         B.replay(ksp)
-    t_plain, ref = _timed(plain, max(1, args.steps // 4))
-    t_batch, got = _timed(lambda: [x.clone() for x in B.replay(ksp)], args.steps)
+    t_plain, ref = _timed(plain, max(1, steps // 4))
+    t_batch, got = _timed(lambda: [x.clone() for x in B.replay(ksp)], steps)
     err = max(float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)) for a, b in zip(got, ref))
     rec = {
         "metric": "CG-SENSE reconstructions/s (64 uneven 2D problems, 10 CG iterations each)",
-        "value": 64 / t_batch, "unit": "recons/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "value": 64 / t_batch, "unit": "recons/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64", "data": "synthetic (seeded radial/spiral trajectories, Gaussian-bump coil maps, phantom k-space + 30 dB noise)",
         "config": {"workload": "64 uneven 2D CG-SENSE problems, one captured graph", "N": [128, 192, 256],
                    "coils": [8, 16, 32], "samples_per_coil": [20000, 130000], "cg_iters": 10, "width": 4,
-                   "oversamp": 1.25, "samples_per_item": str(args.samples_per_item)},
+                   "oversamp": 1.25, "samples_per_item": str(samples_per_item)},
         "plain_loop": {"value": 64 / t_plain, "unit": "recons/s", "ms": t_plain * 1e3,
                        "path": "per problem: sense_linop + CG_SENSE_recon (fused single-problem HIP operators)"},
         "speedup_vs_plain": t_plain / t_batch,
         "max_rel_l2_vs_plain": err,
         "total_samples": int(sum(c * m for _, c, m in spec)),
     }
-    print(json.dumps(rec), flush=True)
+    return rec
 
 
 def run_grad(args):
+    print(json.dumps(measure_grad(args.steps, args.warmup)), flush=True)
+
+
+def _dot_re64(a, b):
+    return torch.real(torch.sum(a.conj() * b, dtype=torch.complex128))
+
+
+def measure_grad(steps, warmup):
     """loss = ||A(k) x - y||^2 + ||CG_10(A(k), y) - x0||^2, 256^2 image, 16 coils, 64 spiral
-    interleaves x 2048; gradients w.r.t. x and the trajectory k."""
+    interleaves x 2048; gradients w.r.t. x and the trajectory k. Both paths reduce the CG
+    inner products and step sizes in FP64."""
     from mr_recon_amd import diff, synth
     from mr_recon_amd.fourier import hip_nufft
 
@@ -256,24 +297,24 @@ def run_grad(args):
         xx = ahb
         r = ahb - (AH(A(xx)) * inv + lam * xx)
         p = r
-        rz = torch.real(torch.sum(r.conj() * r))
+        rz = _dot_re64(r, r)
         for _ in range(iters):
             ap = AH(A(p)) * inv + lam * p
-            a = rz / torch.real(torch.sum(p.conj() * ap))
+            a = (rz / _dot_re64(p, ap)).to(torch.float32)
             xx = xx + a * p
             r = r - a * ap
-            rzn = torch.real(torch.sum(r.conj() * r))
-            p = r + (rzn / rz) * p
+            rzn = _dot_re64(r, r)
+            p = r + (rzn / rz).to(torch.float32) * p
             rz = rzn
         loss = dc + torch.sum(torch.abs(xx - x0) ** 2)
         loss.backward()
         return loss.detach(), x.grad, k.grad
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         fused()
         plain()
-    t_f, (lf, gxf, gkf) = _timed(fused, args.steps)
-    t_p, (lp, gxp, gkp) = _timed(plain, max(1, args.steps // 2))
+    t_f, (lf, gxf, gkf) = _timed(fused, steps)
+    t_p, (lp, gxp, gkp) = _timed(plain, max(1, steps // 2))
 
     def rel(a, b):
         return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
@@ -294,7 +335,7 @@ def run_grad(args):
 
     rec = {
         "metric": "loss+gradient evaluations/s (2D 256^2 x 16 coils SENSE NUFFT + 10-iteration unrolled CG)",
-        "value": 1.0 / t_f, "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "value": 1.0 / t_f, "unit": "evals/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": t_f * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64", "data": "synthetic (seeded spiral, phantom with random phase, Gaussian-bump coil maps)",
         "config": {"workload": "dL/dx and dL/dk of ||A(k)x-y||^2 + ||CG10(A(k),y)-x0||^2", "image": [N, N],
@@ -308,7 +349,7 @@ def run_grad(args):
                       "data_term_only": {"loss": abs(float(ldf - ldp)) / abs(float(ldp)), "grad_x": rel(gxdf, gxdp),
                                          "grad_k": rel(gkdf, gkdp)}},
     }
-    print(json.dumps(rec), flush=True)
+    return rec
 
 
 def run_cg(args, three_d: bool):
@@ -549,10 +590,10 @@ def main():
     ap.add_argument("--workload", default="nufft3d", choices=["nufft3d", "batched", "grad", "cg2d", "cg3d", "dcf", "rows"])
     ap.add_argument("--samples-per-item", default="auto",
                     help="batched workload: work-item size (an int, or 'auto' for the autotuner)")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="after the timed steps, run one elementwise kernel that reads 1 GiB and writes 1 GiB "
-                         "(calibrates the HBM byte counters)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="default workload: skip the batched and gradient forms (profiling runs)")
     args = ap.parse_args()
+    torch.set_num_threads(cpu_threads())
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build runs the NUFFT on a single MI355X "
               "(multi-GPU sharding is a design note, not implemented). Use --gpus 1.", file=sys.stderr)
@@ -576,10 +617,22 @@ def main():
     dev = torch.device("cuda:0")
     trj, mps, img = make_inputs(dev)
     t0 = time.time()
-    A = sense_linop(trj, mps, nufft=hip_nufft((N, N, N), oversamp=OS, width=WIDTH))
+    nf = hip_nufft((N, N, N), oversamp=OS, width=WIDTH)
+    A = sense_linop(trj, mps, nufft=nf)
     torch.cuda.synchronize()
-    log(f"plan built in {time.time() - t0:.2f}s; M={A.plan.npts} per coil, items={A.plan.max_items}")
     M = A.plan.npts
+    log(f"operator built in {time.time() - t0:.2f}s; M={M} per coil, {A.plan.nitems} work items")
+    # the per-trajectory plan (binning, sort, KB weights) on its own: a fresh trajectory
+    # tensor misses the plan cache
+    plan_ms = []
+    for _ in range(3):
+        t_new = trj.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nf.plan(t_new.reshape(-1, 3))
+        torch.cuda.synchronize()
+        plan_ms.append((time.perf_counter() - t0) * 1e3)
+        del t_new
 
     for _ in range(args.warmup):
         A.adjoint(A.forward(img))
@@ -623,6 +676,8 @@ def main():
                    "grid": [math.ceil(OS * N)] * 3},
         "step_GBps_algorithmic": step_bytes(M) / (ms * 1e-3) / 1e9,
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+        "plan_build_ms": round(min(plan_ms), 3),
+        "work_items": int(A.plan.nitems),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "roofline_detail": {"kernel": dom, "avg_ms": dom_ms, "launches_per_step": launches[dom],
@@ -635,13 +690,23 @@ def main():
     if traffic is not None:
         rec["roofline"]["traffic"] = traffic["bytes"]
         rec["roofline_detail"]["traffic_source"] = traffic["source"]
-    if args.copy_probe:
-        src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        torch.mul(src, 1.0, out=dst)  # a copy through the shader cores (a D2D memcpy may use DMA)
-        torch.cuda.synchronize()
-        del src, dst
+    del A, trj, mps, img
+    torch.cuda.empty_cache()
+    if not args.headline_only:
+        # the other two halves of the metric: batched and trajectory-gradient forms against
+        # the plain per-problem path on this GPU (each reports its agreement with it)
+        t0 = time.time()
+        b = measure_batched(steps=8, warmup=1)
+        rec["batched"] = {k: b[k] for k in ("value", "unit", "ms_per_step", "plain_loop", "speedup_vs_plain",
+                                            "max_rel_l2_vs_plain", "total_samples")}
+        rec["batched"]["workload"] = b["config"]["workload"]
+        torch.cuda.empty_cache()
+        g = measure_grad(steps=8, warmup=1)
+        rec["grad"] = {k: g[k] for k in ("value", "unit", "ms_per_step", "plain_path", "speedup_vs_plain",
+                                         "agreement")}
+        rec["grad"]["workload"] = g["config"]["workload"]
+        torch.cuda.empty_cache()
+        log(f"batched and gradient forms took {time.time() - t0:.1f}s")
     if not args.no_cpu_baseline:
         t0 = time.time()
         rec["cpu_baseline"] = cpu_baseline()

@@ -157,15 +157,21 @@ def sense_normal(x, trj, mps, nufft: hip_nufft, dcf=None):
     return sense_adjoint(sense_forward(x, trj, mps, nufft), trj, mps, nufft, dcf)
 
 
+def _dot_re64(a, b):
+    """Re <a, b> summed in FP64 (complex128 accumulation of the complex64 products)."""
+    return torch.real(torch.sum(a.conj() * b, dtype=torch.complex128))
+
+
 def _cg_iter(x, r, p, rz, trj, mps, nufft, dcf, lamda, inv_eig):
-    """One CG step (algs.py:779-872 with P = identity) as differentiable torch ops."""
+    """One CG step (algs.py:779-872 with P = identity) as differentiable torch ops; the
+    inner products and the step sizes alpha, beta are FP64 scalars."""
     ap = sense_normal(p, trj, mps, nufft, dcf) * inv_eig + lamda * p
-    pap = torch.real(torch.sum(p.conj() * ap))
-    alpha = rz / pap
+    pap = _dot_re64(p, ap)
+    alpha = (rz / pap).to(torch.float32)
     x = x + alpha * p
     r = r - alpha * ap
-    rz_new = torch.real(torch.sum(r.conj() * r))
-    p = r + (rz_new / rz) * p
+    rz_new = _dot_re64(r, r)
+    p = r + (rz_new / rz).to(torch.float32) * p
     return x, r, p, rz_new
 
 
@@ -184,7 +190,7 @@ def unrolled_cg_sense(ksp, trj, mps, nufft: hip_nufft, num_iters: int = 10, lamd
         return x
     r = ahb - (sense_normal(x, trj, mps, nufft, dcf) * inv + lamda_l2 * x)
     p = r
-    rz = torch.real(torch.sum(r.conj() * r))
+    rz = _dot_re64(r, r)
     for _ in range(num_iters):
         if checkpoint:
             x, r, p, rz = ckpt.checkpoint(_cg_iter, x, r, p, rz, trj, mps, nufft, dcf, lamda_l2, inv,

@@ -473,6 +473,7 @@ class gridded_nufft(NUFFT):
         super().__init__(im_size)
         self.im_size_os = tuple(round(i * oversamp) for i in self.im_size)
         self.oversamp = oversamp
+        self._tables = _PlanCache()
 
     def rescale_trajectory(self, trj: torch.Tensor) -> torch.Tensor:
         """k -> round(k * oversamp + G/2) mod G as int32 (fourier.py:1222-1233)."""
@@ -483,6 +484,18 @@ class gridded_nufft(NUFFT):
         return trj_rs.type(torch.int32)
 
     def _flat_index(self, trj_n: torch.Tensor, G, unshifted: bool = False) -> torch.Tensor:
+        """Flat index of each sample's grid point (cached per trajectory tensor)."""
+        return self._tables.get(trj_n, ("idx", tuple(G), unshifted),
+                                lambda: self._build_flat_index(trj_n, G, unshifted))
+
+    def _scatter_tables(self, trj_n: torch.Tensor, G):
+        """Sorted-chunk tables of the fused adjoint's scatter (cached per trajectory tensor):
+        built once, so repeated adjoints on one trajectory run no sort and no host sync."""
+        return self._tables.get(trj_n, ("chunks", tuple(G)),
+                                lambda: self._chunks(self._flat_index(trj_n, G, unshifted=True)))
+
+    @staticmethod
+    def _build_flat_index(trj_n: torch.Tensor, G, unshifted: bool = False) -> torch.Tensor:
         """Flat index of each sample's grid point; indices follow Python's rule (-G..G-1).
         ``unshifted``: position in the unshifted FFT layout, where centred index i sits
         at (i - G//2) mod G."""
@@ -607,7 +620,7 @@ class gridded_nufft(NUFFT):
         out = torch.empty((N, B, *self.im_size), dtype=complex_dtype, device=ksp.device)
         for n in range(N):
             y = ksp[n].reshape(B, M).to(complex_dtype).contiguous()
-            perm, cstart, cend, cidx = self._chunks(self._flat_index(trj[n], G, unshifted=True))
+            perm, cstart, cend, cidx = self._scatter_tables(trj[n], G)
             g = torch.zeros((B, *G), dtype=complex_dtype, device=ksp.device)
             _lib.check(_lib.lib().mra_nn_scatter_chunks(_lib.ptr(y), _lib.ptr(perm), _lib.ptr(cstart),
                                                         _lib.ptr(cend), _lib.ptr(cidx), cidx.numel(), M, B,

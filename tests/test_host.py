@@ -170,3 +170,29 @@ def test_lazy_conj_views_are_refused():
     y = _c64(x.conj(), "x")
     assert not y.is_conj() and torch.equal(y, torch.conj_physical(x))
     assert _lib.ptr(x) is not None
+
+
+def test_gridded_scatter_chunks_vs_groupby():
+    """gridded_nufft._chunks (torch ops, run here on the CPU) against a brute-force group-by of
+    the sorted indices: chunks tile every run in order, none longer than the chunk size, each
+    names its run's index, and the permutation is the stable sort."""
+    import itertools
+
+    from mr_recon_amd.fourier import gridded_nufft
+
+    g = torch.Generator().manual_seed(0)
+    idx = torch.cat([torch.randint(0, 50, (500,), generator=g), torch.full((300,), 7), torch.full((129,), 3),
+                     torch.full((64,), 11), torch.full((65,), 12)]).to(torch.int32)
+    idx = idx[torch.randperm(idx.numel(), generator=g)]
+    for chunk in (1, 4, 64):
+        perm, cs, ce, ci = gridded_nufft._chunks(idx, chunk)
+        order = sorted(range(idx.numel()), key=lambda i: (int(idx[i]), i))
+        assert perm.tolist() == order
+        want = []
+        pos = 0
+        for key, grp in itertools.groupby([int(idx[i]) for i in order]):
+            n = len(list(grp))
+            for s in range(0, n, chunk):
+                want.append((pos + s, pos + min(s + chunk, n), key))
+            pos += n
+        assert list(zip(cs.tolist(), ce.tolist(), ci.tolist())) == want

@@ -231,6 +231,44 @@ def test_sense_identities_and_cg():
     assert rel_l2(xs, torch.linalg.solve(Hm, b)) < 1e-9
 
 
+def test_power_method_vs_eigvalsh():
+    """power_method_operator (algs.py:364-399) returns the largest eigenvalue of a Hermitian
+    PSD operator (torch.linalg.eigvalsh) and a unit vector along its eigenvector; the top
+    eigenvalue is separated (ratio 0.2) so 15 iterations converge to ~1e-10."""
+    from oracle.sense_ref import cg_sense_recon, power_method_operator
+
+    g = torch.Generator().manual_seed(4)
+    Q, _ = torch.linalg.qr(torch.randn(40, 40, dtype=torch.complex128, generator=g))
+    lam = torch.cat([torch.tensor([7.5]), torch.linspace(1.5, 0.01, 39)]).to(torch.float64)
+    H = (Q * lam) @ Q.conj().T
+    x0 = torch.randn(40, dtype=torch.complex128, generator=g)
+    v, ll = power_method_operator(lambda u: H @ u, x0)
+    ev = torch.linalg.eigvalsh(H)
+    assert abs(ll - float(ev[-1])) < 1e-9 * float(ev[-1])
+    assert abs(float(torch.linalg.vector_norm(v)) - 1.0) < 1e-12
+    assert abs(abs(complex(torch.vdot(Q[:, 0], v))) - 1.0) < 1e-9
+    # a non-normalising iteration or the norm of x0 instead of A x0 misses the eigenvalue
+    assert abs(float(torch.linalg.vector_norm(H @ x0)) - float(ev[-1])) > 1e-3
+
+    # CG_SENSE_recon with max_eigen=None (recons.py:119-123) scales the system by 1.01 lambda_max
+    class Dense:
+        ishape = (40,)
+        cdt = torch.complex128
+
+        def normal(self, u):
+            return H @ u
+
+        def adjoint(self, y):
+            return y
+
+    b = torch.randn(40, dtype=torch.complex128, generator=g)
+    torch.manual_seed(0)
+    rec = cg_sense_recon(Dense(), b, max_iter=200, lamda_l2=0.0, max_eigen=None, tolerance=1e-14)
+    s = 1.01 * float(ev[-1])
+    exact = torch.linalg.solve(H / s, b / s ** 0.5)
+    assert rel_l2(rec, exact) < 1e-9
+
+
 def test_oracle_autograd_matches_fp64_finite_differences():
     """Autograd through the restated pure-torch path (NaN-safe masking) gives the trajectory
     gradient that FP64 central differences of the same loss give."""
