@@ -77,6 +77,20 @@ int mra_grid_partial(int ndim, int P, const void* ksp, const float* dcf, const f
                      void* scratch, void* stream);
 int mra_grid_reduce(int ndim, int P, void* grid, const void* scratch, const void* geoms, int prob0, int nprob,
                     const int* tile_items, int ntiles, int nbatch, int accumulate, void* stream);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops): mra_grid_partial + mra_grid_reduce of one
+ * problem whose phase-1 subgrids live in a static 64 MiB device array of the library (one
+ * per GPU, held until the process ends) instead of a caller buffer. `geom_host` is the
+ * problem's host Geom record (mra_geom_fill, item_base = scratch_off = tile_items_off = 0);
+ * the call copies it to the device with a kernel and allocates nothing. Fails (status 1,
+ * message in mra_last_error) when nitems x nbatch x mra_subgrid_points(ndim, P) x 8 bytes
+ * exceeds mra_grid_noalloc_max_bytes(). The static scratch is shared by every call on one
+ * GPU: calls that overlap on one GPU (on different streams, or from several host threads)
+ * race and must be serialised by the caller. The stream must belong to the current device. */
+int mra_grid_noalloc(int ndim, int P, const void* ksp, const float* dcf, const float* wsort, const int* lbase,
+                     const int* perm, const void* geom_host, const void* items, int nitems, int nbatch,
+                     const int* tile_items, int ntiles, void* grid, int accumulate, void* stream);
+long long mra_grid_noalloc_max_bytes(void);
 /* Marks a host Geom record (from mra_geom_fill) so that the gridding kernels index dcf by
  * sorted (plan) position, dcf_sorted[s] = dcf[perm[s]], instead of through perm. No
  * reference counterpart (the reference multiplies ksp by dcf before gridding,

@@ -51,6 +51,8 @@ SIGNATURES = {
     "mra_interp": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "mra_grid_partial": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp]),
     "mra_grid_reduce": (_i, [_i, _i, _vp, _vp, _vp, _i, _i, _vp, _i, _i, _i, _vp]),
+    "mra_grid_noalloc": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _i, _vp, _i, _vp]),
+    "mra_grid_noalloc_max_bytes": (_ll, []),
     "mra_interp_grad": (_i, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
     "mra_pad_expand": (_i, [_i, _ip, _ip, _ip, _vp, _ll, _ll, _vp, _ll, _ll, _vp, _f, _vp, _vp]),
     "mra_crop_reduce": (_i, [_i, _ip, _ip, _ip, _vp, _ll, _vp, _ll, _ll, _i, _vp, _f, _i, _i, _vp, _ll, _vp]),

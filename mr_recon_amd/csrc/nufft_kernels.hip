@@ -26,6 +26,7 @@
 //                and writes the grid once. Device-scope float atomics run at a
 //                few G/s on MI355X, so gridding never uses them.
 //  interp_grad   derivative weights, reduced over coils per sample.
+#include <mutex>
 #include <type_traits>
 
 #include "common.h"
@@ -1331,11 +1332,115 @@ static int subgrid_points(int ndim, int P) {
   return ipow(T + P - 1, ndim);
 }
 
+// ---- gridding without device allocation ----------------------------------------------------
+// Callers that cannot allocate device memory (code under HIP graph capture, serving loops)
+// grid through mra_grid_noalloc, whose scratch is this file-scope device array: a Geom slot
+// followed by the per-(work item, batch entry) subgrids of phase 1. Every GPU the library
+// runs on holds its own copy until the process ends. Host code never passes the array's
+// name to a kernel: it uses the address hipGetSymbolAddress returns on each device.
+constexpr size_t kNoAllocBytes = size_t(64) << 20;  // 64 MiB per GPU
+constexpr size_t kNoAllocGeomBytes = 256;           // Geom slot at the start (keeps subgrids 256-byte aligned)
+static_assert(sizeof(Geom) <= kNoAllocGeomBytes, "Geom slot");
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[kNoAllocMaxDevices];
+
+// Device address of `scratch` on the current device (looked up once per device).
+static int noalloc_scratch(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocMaxDevices) {
+    set_last_error("grid_noalloc: no current HIP device");
+    return 1;
+  }
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      char b[256];
+      snprintf(b, sizeof(b), "grid_noalloc: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      set_last_error(b);
+      return 1;
+    }
+    g_noalloc_ptr[dev] = p;
+  }
+  *out = g_noalloc_ptr[dev];
+  return 0;
+}
+
+__global__ void store_geom_kernel(Geom g, Geom* __restrict__ dst) { *dst = g; }
+
 }  // namespace mra
 
 using namespace mra;
 
 extern "C" {
+
+long long mra_grid_noalloc_max_bytes() { return (long long)(kNoAllocBytes - kNoAllocGeomBytes); }
+
+// Gridding (phase 1 + reduce, the kernels of mra_grid_partial / mra_grid_reduce) of one
+// problem with the library's static scratch instead of a caller buffer. `geom_host` is the
+// problem's host Geom record (mra_geom_fill); it is copied into the scratch's Geom slot by a
+// kernel, so the call makes no allocation and no host-to-device copy.
+int mra_grid_noalloc(int ndim, int P, const void* ksp, const float* dcf, const float* wsort, const int* lbase,
+                     const int* perm, const void* geom_host, const void* items, int nitems, int nbatch,
+                     const int* tile_items, int ntiles, void* grid, int accumulate, void* stream) {
+  MRA_TRY_BEGIN
+  if (!geom_host) MRA_FAIL("grid_noalloc: null Geom record");
+  const Geom g = *(const Geom*)geom_host;
+  if (ndim < 1 || ndim > kMaxDim) MRA_FAIL("grid_noalloc: ndim must be 1..3");
+  if (nbatch < 1 || g.nbatch != nbatch) MRA_FAIL("grid_noalloc: nbatch must be positive and match the Geom record");
+  if (g.item_base != 0 || g.scratch_off != 0 || g.tile_items_off != 0)
+    MRA_FAIL("grid_noalloc: one problem per call (item_base, scratch_off and tile_items_off must be 0)");
+  long long nt = 1;
+  for (int d = 0; d < ndim; ++d) nt *= g.ntile[d];
+  if (ntiles != nt) MRA_FAIL("grid_noalloc: ntiles does not match the Geom record");
+  if (nitems < 0) MRA_FAIL("grid_noalloc: nitems must be non-negative");
+  const long long need = (long long)nitems * nbatch * subgrid_points(ndim, P) * (long long)sizeof(float2);
+  if (need > mra_grid_noalloc_max_bytes()) {
+    char b[256];
+    snprintf(b, sizeof(b), "grid_noalloc: needs %lld bytes of scratch (items x batch x subgrid), the static "
+             "scratch holds %lld; use mra_grid_partial with a caller buffer", need, mra_grid_noalloc_max_bytes());
+    MRA_FAIL(b);
+  }
+  void* base = nullptr;
+  if (noalloc_scratch(&base)) return 1;
+  Geom* gslot = (Geom*)base;
+  float2* sub = (float2*)((unsigned char*)base + kNoAllocGeomBytes);
+  const hipStream_t st = (hipStream_t)stream;
+  store_geom_kernel<<<1, 1, 0, st>>>(g, gslot);
+  MRA_CHECK_LAUNCH("grid_noalloc: Geom slot");
+  LaunchArgs a{};
+  a.op = Op::GridPartial;
+  a.src = ksp;
+  a.aux = dcf;
+  a.dst = sub;
+  a.wsort = wsort;
+  a.lbase = lbase;
+  a.perm = perm;
+  a.geoms = gslot;
+  a.items = (const Item*)items;
+  a.nitems = nitems;
+  a.nbatch = nbatch;
+  a.stream = st;
+  if (const int rc = dispatch(ndim, P, a)) return rc;
+  LaunchArgs r{};
+  r.op = Op::GridReduce;
+  r.src = sub;
+  r.dst = grid;
+  r.geoms = gslot;
+  r.prob = 0;
+  r.nprob = 1;
+  r.tile_items = tile_items;
+  r.ntiles = ntiles;
+  r.nbatch = nbatch;
+  r.accumulate = accumulate;
+  r.stream = st;
+  return dispatch(ndim, P, r);
+  MRA_TRY_END
+}
 
 int mra_geom_size() { return (int)sizeof(Geom); }
 

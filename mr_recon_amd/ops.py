@@ -111,6 +111,31 @@ def gridding_reduce(scratch: torch.Tensor, plan: SamplePlan, out: torch.Tensor,
     return out
 
 
+def gridding_noalloc(ksp: torch.Tensor, plan: SamplePlan, out: torch.Tensor, dcf: torch.Tensor | None = None,
+                     out_scale: float = 1.0, accumulate: bool = False) -> torch.Tensor:
+    """``gridding`` without any device allocation: the phase-1 subgrids go to the library's
+    static scratch (``mra_grid_noalloc``), so the call can run under HIP graph capture. Raises
+    when the plan needs more scratch than that array holds. Calls that overlap on one GPU
+    share the array and must not run concurrently."""
+    import ctypes
+
+    _lib.require_device(ksp, dcf, out)
+    ksp = _c64(ksp, "ksp")
+    B = _grid_batch(plan, out)
+    if ksp.numel() != B * plan.npts:
+        raise ValueError(f"gridding: ksp has {ksp.numel()} elements, expected {B}x{plan.npts}")
+    if dcf is not None:
+        if dcf.dtype != torch.float32 or dcf.numel() != plan.npts:
+            raise ValueError("dcf must be float32 with one entry per sample")
+        dcf = dcf.contiguous()
+    gh = plan.geom_host(B, out_scale)
+    check(lib().mra_grid_noalloc(plan.ndim, plan.P, ptr(ksp), ptr(dcf), ptr(plan.wsort), ptr(plan.lbase),
+                                 ptr(plan.perm), ctypes.addressof(gh), ptr(plan.items), plan.nitems, B,
+                                 ptr(plan.tile_items), plan.ntiles, ptr(out), int(accumulate),
+                                 stream_ptr(out.device)))
+    return out
+
+
 def gridding_scratch_elems(plan: SamplePlan, nbatch: int) -> int:
     """complex64 elements of gridding scratch: one subgrid per (work item, batch entry)."""
     return max(1, plan.nitems * nbatch * plan.subgrid)

@@ -172,8 +172,17 @@ class SamplePlan:
         return g
 
 
-def make_geoms(entries: list[dict], device) -> torch.Tensor:
-    """Pack Geom records (see common.h) into a device byte tensor."""
+    def geom_host(self, nbatch: int, out_scale: float = 1.0):
+        """Host Geom record of a one-problem call on ``nbatch`` grids (what ``geom`` puts in
+        device memory), for the no-allocation gridding entry point."""
+        return pack_geoms([dict(grid_off=0, grid_stride=math.prod(self.grid_size), ksp_off=0, ksp_stride=self.npts,
+                                samp_off=0, nbatch=int(nbatch), npts=self.npts, grid_size=self.grid_size,
+                                tile=self.tile, wrap=self.wrap, beta=self.beta, width=self.width,
+                                out_scale=out_scale)])
+
+
+def pack_geoms(entries: list[dict]):
+    """Pack Geom records (see common.h) into a host ctypes byte buffer."""
     L = lib()
     gsz = L.mra_geom_size()
     buf = (C.c_ubyte * (gsz * len(entries)))()
@@ -187,5 +196,10 @@ def make_geoms(entries: list[dict], device) -> torch.Tensor:
             check(L.mra_geom_set_dcf_plan_order(C.addressof(buf) + i * gsz, 1))
         if e.get("ksp_plan_order"):
             check(L.mra_geom_set_ksp_plan_order(C.addressof(buf) + i * gsz, 1))
-    host = torch.frombuffer(bytearray(bytes(buf)), dtype=torch.uint8)
+    return buf
+
+
+def make_geoms(entries: list[dict], device) -> torch.Tensor:
+    """Pack Geom records (see common.h) into a device byte tensor."""
+    host = torch.frombuffer(bytearray(bytes(pack_geoms(entries))), dtype=torch.uint8)
     return host.to(device)
