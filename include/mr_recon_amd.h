@@ -184,6 +184,38 @@ int mra_nn_scatter(const void* ksp, const int* idx, long long M, int B, float sc
 int mra_nn_scatter_chunks(const void* ksp, const int* perm, const int* cstart, const int* cend, const int* cidx,
                           long long NC, long long M, int B, float scale, void* grid, long long gstride,
                           void* stream);
+/* Low-rank forms of the nearest-grid transform, chebyshev_nufft.forward / adjoint
+ * (fourier.py:1703-1774: multi_index of L spectra times the temporal functions h, summed
+ * over L; multi_grid of k-space times conj(h)). grid holds B * L grids of gstride points,
+ * h is (L, M) complex64. gather_sum: out[b, m] = scale * sum_l h[l, m] grid[b L + l, idx[m]].
+ * scatter_sum_chunks: adds scale * sum_{s in chunk c} conj(h[l, s]) ksp[b, s] into
+ * grid[b L + l, cidx[c]] (chunk tables as for mra_nn_scatter_chunks). */
+int mra_nn_gather_sum(const void* grid, long long gstride, const int* idx, const void* h, int L, long long M, int B,
+                      float scale, void* out, void* stream);
+int mra_nn_scatter_sum_chunks(const void* ksp, const void* h, int L, const int* perm, const int* cstart,
+                              const int* cend, const int* cidx, long long NC, long long M, int B, float scale,
+                              void* grid, long long gstride, void* stream);
+
+/* ---- 4D and 5D interpolation / gridding ------------------------------------------------------
+ * ungrid / grid for 4- and 5-dimensional grids: the Triton kernels _ungrid4d / _ungrid5d
+ * (triton_interp/ungrid.py:498-845) and grid's torch fallback there (grid.py:86-127, 556-624).
+ * t: M x ndim sample coordinates in grid units, already wrapped to [0, G] (prep_ungrid_shapes);
+ * G, betas: ndim entries; P = floor(width)+1 in 2..7; every weight times out_scale. interp:
+ * out[b, m] = sum over the patch of KB weight x grid[b, g] (B grids of prod(G) points); grid:
+ * adds the samples into grid with float atomics (the caller zeroes it). */
+int mra_interp_direct(int ndim, int P, const void* grid, const float* t, long long M, int B, const int* G,
+                      const float* betas, float width, float out_scale, void* out, void* stream);
+int mra_grid_direct(int ndim, int P, const void* ksp, const float* t, long long M, int B, const int* G,
+                    const float* betas, float width, float out_scale, void* grid, void* stream);
+
+/* ---- low-rank coil / imperfection mixing ---------------------------------------------------
+ * imperf_coil_lowrank.forward / adjoint (linops.py:614-714): the einsums 'L ..., L C ... -> C ...'
+ * and 'C ..., L C ... -> L ...' with conj(h) and the dcf, h = temporal functions (L, C, M)
+ * complex64. expand: out[c, m] = sum_l h[l, c, m] y[l, m]; reduce: out[l, m] = dcf[m] *
+ * sum_c conj(h[l, c, m]) ksp[c, m] (dcf may be NULL). */
+int mra_lowrank_expand(const void* y, const void* h, int L, int C, long long M, void* out, void* stream);
+int mra_lowrank_reduce(const void* ksp, const void* h, const float* dcf, int L, int C, long long M, void* out,
+                       void* stream);
 
 /* ---- density compensation ---------------------------------------------------------------
  * Elementwise steps of the Pipe-Menon iteration behind density_compensation (algs.py:20-106,

@@ -81,6 +81,12 @@ SIGNATURES = {
     "mra_nn_gather": (_i, [_vp, _ll, _vp, _ll, _i, _f, _vp, _vp]),
     "mra_nn_scatter": (_i, [_vp, _vp, _ll, _i, _f, _vp, _ll, _vp]),
     "mra_nn_scatter_chunks": (_i, [_vp, _vp, _vp, _vp, _vp, _ll, _ll, _i, _f, _vp, _ll, _vp]),
+    "mra_nn_gather_sum": (_i, [_vp, _ll, _vp, _vp, _i, _ll, _i, _f, _vp, _vp]),
+    "mra_nn_scatter_sum_chunks": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _ll, _ll, _i, _f, _vp, _ll, _vp]),
+    "mra_interp_direct": (_i, [_i, _i, _vp, _vp, _ll, _i, _ip, _fp, _f, _f, _vp, _vp]),
+    "mra_grid_direct": (_i, [_i, _i, _vp, _vp, _ll, _i, _ip, _fp, _f, _f, _vp, _vp]),
+    "mra_lowrank_expand": (_i, [_vp, _vp, _i, _i, _ll, _vp, _vp]),
+    "mra_lowrank_reduce": (_i, [_vp, _vp, _vp, _i, _i, _ll, _vp, _vp]),
     "mra_dcf_partials": (_i, []),
     "mra_dcf_update": (_i, [_vp, _vp, _ll, _vp]),
     "mra_dcf_normalize": (_i, [_vp, _vp, _vp, _ll, _vp, _vp]),

@@ -255,10 +255,51 @@ class hip_nufft(NUFFT):
         return self._plans.get(t, (G, wrap, im_size, self.width, self.beta, self.samples_per_item), build)
 
     # -- split methods (reference layout) -------------------------------
+    # -- 4D / 5D: the reference's composition with torch FFTs and the direct KB kernels -------
+    def _scaled_coords(self, trj_n: torch.Tensor, im_size) -> torch.Tensor:
+        """_scale_trj (fourier.py:420-430): t = fl32(fl32(k * fl32(G/N)) + G//2), per dim."""
+        sc, sh = self._scale_shift(im_size)
+        t = trj_n.to(torch.float32)
+        return (t * torch.tensor(sc, dtype=torch.float32, device=t.device)) + torch.tensor(
+            sh, dtype=torch.float32, device=t.device)
+
+    def _direct(self, adjoint: bool, v: torch.Tensor, t: torch.Tensor, G, nbatch: int) -> torch.Tensor:
+        d = len(G)
+        M = int(math.prod(t.shape[:-1]))
+        tw = torch.remainder(t.reshape(-1, d), torch.tensor(G, dtype=torch.float32, device=t.device)).contiguous()
+        P = int(math.floor(self.width)) + 1
+        args = (d, P, _lib.ptr(v), _lib.ptr(tw), M, nbatch, _lib.ints(G), _lib.floats([self.beta] * d),
+                float(self.width), 1.0 / self.width ** d)
+        if adjoint:
+            out = torch.zeros((nbatch, *G), dtype=complex_dtype, device=v.device)
+            _lib.check(_lib.lib().mra_grid_direct(*args, _lib.ptr(out), _lib.stream_ptr(v.device)))
+        else:
+            out = torch.empty((nbatch, M), dtype=complex_dtype, device=v.device)
+            _lib.check(_lib.lib().mra_interp_direct(*args, _lib.ptr(out), _lib.stream_ptr(v.device)))
+        return out
+
+    def _forward_FT_generic(self, img: torch.Tensor) -> torch.Tensor:
+        d = len(self.im_size)
+        x = img.to(complex_dtype).clone()
+        if self.apodize:
+            x = self.apodize_img(x)
+        x = x / math.prod(x.shape[-d:]) ** 0.5
+        x = resize(x, tuple(x.shape[:-d]) + self.os_shape(tuple(x.shape[-d:])))
+        return fft(x, dim=tuple(range(-d, 0)), norm=None)
+
+    def _adjoint_iFT_generic(self, ksp_os: torch.Tensor) -> torch.Tensor:
+        d = len(self.im_size)
+        G = tuple(ksp_os.shape[-d:])
+        img = ifft(ksp_os.to(complex_dtype), dim=tuple(range(-d, 0)), norm=None)
+        img = resize(img, tuple(img.shape[:-d]) + self.im_size) * (math.prod(G) / math.prod(self.im_size) ** 0.5)
+        return self.apodize_img(img) if self.apodize else img
+
     def forward_FT_only(self, img: torch.Tensor) -> torch.Tensor:
         """apodize -> / sqrt(prod N) -> centred zero-pad -> centred FFT (norm=None)."""
         _lib.require_device(img)
         d = len(self.im_size)
+        if d > 3:
+            return self._forward_FT_generic(img)
         im = tuple(img.shape[-d:])
         G = self.os_shape(im)
         lead = img.shape[:-d]
@@ -279,6 +320,11 @@ class hip_nufft(NUFFT):
         M = int(math.prod(trj_batch))
         out = torch.empty((N, B, M), dtype=complex_dtype, device=ksp_os.device)
         src = ksp_os.to(complex_dtype).reshape(N, B, *G)
+        if d > 3:
+            im = (N, *lead, *self.im_size)
+            for i in range(N):
+                out[i] = self._direct(False, src[i].contiguous(), self._scaled_coords(trj[i], im[-d:]), G, B)
+            return out.reshape(N, *lead, *trj_batch)
         for i in range(N):
             plan = self.plan(trj[i], grid_size=G, centred=True)
             ops.interp(src[i], plan, out_scale=1.0 / self.width ** d, out=out[i])
@@ -296,6 +342,10 @@ class hip_nufft(NUFFT):
         G = self.os_shape()
         out = torch.empty((N, B, *G), dtype=complex_dtype, device=ksp.device)
         src = ksp.to(complex_dtype).reshape(N, B, M)
+        if d > 3:
+            for i in range(N):
+                out[i] = self._direct(True, src[i].contiguous(), self._scaled_coords(trj[i], self.im_size), G, B)
+            return out.reshape(N, *lead, *G)
         for i in range(N):
             plan = self.plan(trj[i], grid_size=G, centred=True)
             ops.gridding(src[i], plan, out[i], out_scale=1.0 / self.width ** d)
@@ -305,6 +355,8 @@ class hip_nufft(NUFFT):
         """centred iFFT (norm=None) -> crop -> * prod(G)/sqrt(prod N) -> apodize."""
         _lib.require_device(ksp_os)
         d = len(self.im_size)
+        if d > 3:
+            return self._adjoint_iFT_generic(ksp_os)
         G = tuple(ksp_os.shape[-d:])
         lead = ksp_os.shape[:-d]
         g = fft_torch.ifftshift(ksp_os.to(complex_dtype), dim=tuple(range(-d, 0))).contiguous()
@@ -318,6 +370,8 @@ class hip_nufft(NUFFT):
         """img (N, *img_batch, *im_size), trj (N, *trj_batch, d) -> ksp (N, *img_batch, *trj_batch)."""
         _lib.require_device(img, trj)
         d = len(self.im_size)
+        if d > 3:
+            return self.forward_interp_only(self.forward_FT_only(img), trj)
         N = trj.shape[0]
         img_batch = img.shape[1:-d]
         B = int(math.prod(img_batch))
@@ -338,6 +392,8 @@ class hip_nufft(NUFFT):
         """ksp (N, *ksp_batch, *trj_batch), trj (N, *trj_batch, d) -> img (N, *ksp_batch, *im_size)."""
         _lib.require_device(ksp, trj)
         d = len(self.im_size)
+        if d > 3:
+            return self.adjoint_iFT_only(self.adjoint_grid_only(ksp, trj))
         N = trj.shape[0]
         trj_batch = trj.shape[1:-1]
         M = int(math.prod(trj_batch))
@@ -639,6 +695,304 @@ class gridded_nufft(NUFFT):
             os_factor ** len(self.im_size))
 
 
+class _LowRankNNUFFT(NUFFT):
+    """Nearest-grid transforms with a low-rank correction of the grid-deviation phase:
+    forward = s * sum_l h_l * multi_index(FFT(pad(b_l x))), adjoint = s * sum_l conj(b_l)
+    crop(iFFT(multi_grid(conj(h_l) y))), with temporal functions ``h`` (L, *trj_batch),
+    spatial functions ``b`` (L, *im_size) and s = ``grid_oversamp`` (the common form of
+    chebyshev_nufft and svd_nufft, fourier.py:1487-1572, 1703-1774). The L spectra of one image
+    come from one fused pad+FFT launch over L maps (b); the weighted gather and the chunked
+    weighted scatter are ``mra_nn_gather_sum`` / ``mra_nn_scatter_sum_chunks``; the adjoint's L
+    inverse transforms and the sum with conj(b) are one fused iFFT+crop launch. All L terms
+    are applied in one call (``n_batch_size`` is kept for interface parity)."""
+
+    def _padder(self) -> PadLast:
+        raise NotImplementedError
+
+    def _index(self, trj_n: torch.Tensor, G):
+        return self._tables.get(trj_n, ("idx", tuple(G)),
+                                lambda: gridded_nufft._build_flat_index(trj_n, G, unshifted=True))
+
+    def _scatter_tables(self, trj_n: torch.Tensor, G):
+        return self._tables.get(trj_n, ("chunks", tuple(G)),
+                                lambda: gridded_nufft._chunks(self._index(trj_n, G)))
+
+    def _check(self, trj):
+        if getattr(self, "h", None) is None:
+            raise RuntimeError(f"{type(self).__name__}: call rescale_trajectory(trj) first (it builds the "
+                               "basis functions)")
+        M = int(math.prod(trj.shape[1:-1]))
+        if self.h.shape[0] != self.b.shape[0] or int(math.prod(self.h.shape[1:])) != M:
+            raise ValueError(f"{type(self).__name__}: trajectory does not match the one rescale_trajectory was given")
+
+    def forward(self, img: torch.Tensor, trj: torch.Tensor) -> torch.Tensor:
+        """img (N, *batch, *im_size), trj (N, *trj_batch, d) int32 from rescale_trajectory."""
+        _lib.require_device(img, trj)
+        self._check(trj)
+        d = len(self.im_size)
+        pad = self._padder()
+        G = pad.pad_im_size
+        N = trj.shape[0]
+        batch = tuple(img.shape[1:-d])
+        B = int(math.prod(batch))
+        L = self.h.shape[0]
+        M = int(math.prod(trj.shape[1:-1]))
+        out = torch.empty((N, B, M), dtype=complex_dtype, device=img.device)
+        for n in range(N):
+            x = img[n if img.shape[0] > 1 else 0].reshape(B, *self.im_size).to(complex_dtype)
+            g = ops.pad_fft(x, self.im_size, G, smaps=self.b, scale=1.0 / math.sqrt(math.prod(G)), center=pad.center)
+            idx = self._index(trj[n], G)
+            _lib.check(_lib.lib().mra_nn_gather_sum(_lib.ptr(g), int(math.prod(G)), _lib.ptr(idx), _lib.ptr(self.h),
+                                                    L, M, B, float(self.grid_oversamp), _lib.ptr(out[n]),
+                                                    _lib.stream_ptr(g.device)))
+        return out.reshape(N, *batch, *trj.shape[1:-1])
+
+    def adjoint(self, ksp: torch.Tensor, trj: torch.Tensor) -> torch.Tensor:
+        """ksp (N, *batch, *trj_batch), trj (N, *trj_batch, d) int32 -> (N, *batch, *im_size)."""
+        _lib.require_device(ksp, trj)
+        self._check(trj)
+        d = len(self.im_size)
+        pad = self._padder()
+        G = pad.pad_im_size
+        N = trj.shape[0]
+        nt = trj.ndim - 2
+        batch = tuple(ksp.shape[1:ksp.ndim - nt])
+        B = int(math.prod(batch))
+        L = self.h.shape[0]
+        M = int(math.prod(trj.shape[1:-1]))
+        out = torch.empty((N, B, *self.im_size), dtype=complex_dtype, device=ksp.device)
+        for n in range(N):
+            y = ksp[n].reshape(B, M).to(complex_dtype).contiguous()
+            perm, cstart, cend, cidx = self._scatter_tables(trj[n], G)
+            g = torch.zeros((B * L, *G), dtype=complex_dtype, device=ksp.device)
+            _lib.check(_lib.lib().mra_nn_scatter_sum_chunks(_lib.ptr(y), _lib.ptr(self.h), L, _lib.ptr(perm),
+                                                            _lib.ptr(cstart), _lib.ptr(cend), _lib.ptr(cidx),
+                                                            cidx.numel(), M, B, 1.0, _lib.ptr(g),
+                                                            int(math.prod(G)), _lib.stream_ptr(y.device)))
+            ops.ifft_crop(g, self.im_size, G, B, smaps=self.b, scale=self.grid_oversamp / math.sqrt(math.prod(G)),
+                          out=out[n], center=pad.center)
+        return out.reshape(N, *batch, *self.im_size)
+
+
+class chebyshev_nufft(_LowRankNNUFFT):
+    """NUFFT from a Chebyshev expansion of the grid deviations (fourier.py:1574-1774; after
+    arXiv:1701.04492, Lemma A.3 of Townsend's thesis): each sample is rounded to its nearest
+    point of a ``round(grid_oversamp * N)`` grid and the residual phase
+    exp(-2 pi i (k - k_grid) r) is expanded as sum_l h_l(k) b_l(r), l over
+    ``n_cheby_per_dim``^d products of one-dimensional terms. Forward:
+    sum_l h_l * multi_index(FFT(pad(b_l x))); adjoint: sum_l conj(b_l) crop(iFFT(multi_grid(
+    conj(h_l) y))); both times ``grid_oversamp``.
+
+    ``rescale_trajectory`` computes h for the trajectory it is given and b for the image
+    (as the reference does) and returns the clamped grid indices the other methods take.
+    """
+
+    def __init__(self, im_size: tuple, n_cheby_per_dim: Optional[int] = 5, grid_oversamp: Optional[float] = 1.0,
+                 n_batch_size: Optional[int] = None):
+        super().__init__(im_size)
+        grd_os_size = tuple(round(i * grid_oversamp) for i in self.im_size)
+        self.grid_oversamp = grid_oversamp
+        self.grog_padder = PadLast(grd_os_size, im_size)
+        self.gamma = 1 / (2 * grid_oversamp)
+        self.n_cheby_per_dim = n_cheby_per_dim
+        self.n_batch_size = n_cheby_per_dim ** len(im_size) if n_batch_size is None else n_batch_size
+        self._tables = _PlanCache()
+
+    @staticmethod
+    def cheby_weights(p, r, gamma):
+        """Coefficient of T_r in the expansion of the p-th term (fourier.py:1601-1608):
+        4 i^r J_{(r+p)/2}(-gamma pi / 2) J_{(r-p)/2}(-gamma pi / 2) when r - p is even, else 0."""
+        from scipy.special import jv
+
+        if (abs(p - r) % 2) == 0:
+            return 4 * (1j ** r) * jv((r + p) / 2, -gamma * np.pi / 2) * jv((r - p) / 2, -gamma * np.pi / 2)
+        return 0
+
+    def compute_spatial_funcs(self, im_size: tuple) -> torch.Tensor:
+        """b[i, p](r) = sum_q s_p s_q T_q(2 r_i) c(p, q), s_0 = 1/2, else 1 (fourier.py:1610-1639);
+        (d, n, *im_size) complex64 on the CPU."""
+        n = self.n_cheby_per_dim
+        d = len(im_size)
+        grd = _gen_grd(im_size)
+        b = torch.zeros((d, n, *im_size), dtype=complex_dtype)
+        for i in range(d):
+            r = grd[..., i]
+            for lp in range(n):
+                sp_ = 0.5 if lp == 0 else 1.0
+                for lq in range(n):
+                    sq = 0.5 if lq == 0 else 1.0
+                    b[i, lp] += torch.cos(lq * torch.arccos(2 * r)) * self.cheby_weights(lp, lq, self.gamma) * (sq * sp_)
+        return b
+
+    def compute_temporal_funcs(self, trj: torch.Tensor) -> torch.Tensor:
+        """h[i, p](k) = T_p((k_i - round(k_i s) / s) / gamma) (fourier.py:1641-1664); (d, n, *trj)
+        complex64 on the trajectory's device. The normalised deviation is clamped to [-1, 1]
+        before arccos: mathematically it lies there, but for grid_oversamp != 1 fp32 rounding can
+        put it an ulp outside, where the reference's arccos returns NaN."""
+        n = self.n_cheby_per_dim
+        d = trj.shape[-1]
+        dev = trj.device
+        trj_dev = trj - torch.round(trj * self.grid_oversamp) / self.grid_oversamp
+        h = torch.zeros((d, n, *trj.shape[:-1]), dtype=complex_dtype, device=dev)
+        for i in range(d):
+            k = trj_dev[..., i]
+            for lp in range(n):
+                h[i, lp] = torch.cos(lp * torch.arccos(torch.clamp(k / self.gamma, -1.0, 1.0)))
+        return h
+
+    def rescale_trajectory(self, trj: torch.Tensor) -> torch.Tensor:
+        """Clamped nearest grid indices (fourier.py:1666-1693); also sets the L temporal
+        functions ``h`` (L, *trj_batch) and spatial functions ``b`` (L, *im_size)."""
+        trj_rs = trj * self.grid_oversamp
+        G = self.grog_padder.pad_im_size
+        for i in range(trj_rs.shape[-1]):
+            trj_rs[..., i] = torch.clamp(trj_rs[..., i] + G[i] / 2, 0, G[i] - 1)
+        trj_rs = torch.round(trj_rs).type(torch.int32)
+        h = self.compute_temporal_funcs(trj)
+        b = self.compute_spatial_funcs(self.im_size).to(trj.device)
+        d = trj.shape[-1]
+        h = _outer_terms([h[i] for i in range(d)]).reshape(-1, *trj.shape[:-1])
+        b = _outer_terms([b[i] for i in range(d)]).reshape(-1, *self.im_size)
+        self.h = h.contiguous()
+        self.b = b.contiguous()
+        return trj_rs
+
+    def _padder(self) -> PadLast:
+        return self.grog_padder
+
+
+class svd_nufft(_LowRankNNUFFT):
+    """NUFFT from a low-rank SVD model of the grid deviations (fourier.py:1328-1572): samples
+    are rounded to the nearest point of a ``round(grid_oversamp * N)`` grid (wrapping) and the
+    residual phase exp(-2 pi i dk r) is approximated by the ``n_svd`` leading singular pairs of
+    the matrix exp(-2 pi i k_m r_n) on the ``svd_mx_size`` pixel grid, interpolated to the
+    samples and pixels with cubic splines.
+
+    The reference finds the eigenvectors of mx^H mx by LOBPCG on the full prod(svd_mx_size)^2
+    matrix. That matrix is the Kronecker product of one-dimensional matrices, so here each
+    dimension's eigenpairs come from a small FP64 eigendecomposition and the n_svd largest
+    products are taken (same eigenspaces; a 3D 50^3 model needs no 125 000^2 matrix). When the
+    n_svd-th eigenvalue is degenerate with the next one the reference's operator depends on
+    LOBPCG's random start; here ties are taken in dimension order.
+    """
+
+    def __init__(self, im_size: tuple, grid_oversamp: Optional[float] = 1.0, n_svd: Optional[int] = 5,
+                 n_batch_size: Optional[int] = None, svd_mx_size: Optional[tuple] = None):
+        super().__init__(im_size)
+        os_size = tuple(round(i * grid_oversamp) for i in self.im_size)
+        self.grid_oversamp = grid_oversamp
+        self.padder = PadLast(os_size, im_size)
+        self.n_svd = n_svd
+        self.n_batch_size = n_svd if n_batch_size is None else n_batch_size
+        self.svd_mx_size = (50,) * len(im_size) if svd_mx_size is None else tuple(svd_mx_size)
+        assert len(self.svd_mx_size) == len(im_size), "svd_mx_size must match im_size in length"
+        self._tables = _PlanCache()
+
+    def _padder(self) -> PadLast:
+        return self.padder
+
+    @staticmethod
+    def spatial_interp(spatial_input: torch.Tensor, coords: torch.Tensor, order: Optional[int] = 3,
+                       mode: Optional[str] = "nearest") -> torch.Tensor:
+        """Cubic-spline interpolation of each of the N grids of ``spatial_input`` (N, d0..dK-1)
+        at ``coords`` (*crds, K) in index units -> (N, *crds) (fourier.py:1351-1411, which calls
+        cupyx map_coordinates(order=3, mode='nearest')). The spline coefficients are
+        scipy.ndimage.spline_filter of the input padded by 12 edge values (what map_coordinates
+        does for mode 'nearest'), computed in FP64 on the host; the 4^K-tap evaluation at the
+        coordinates runs on the coordinates' device."""
+        import numpy as _np
+        from scipy.ndimage import spline_filter
+
+        if order != 3 or mode != "nearest":
+            raise NotImplementedError("spatial_interp implements order=3, mode='nearest' (the reference's call)")
+        K = coords.shape[-1]
+        if spatial_input.ndim != K + 1:
+            raise ValueError("Input tensor must have K spatial dimensions.")
+        npad = 12
+        a = spatial_input.detach().cpu().numpy().astype(_np.complex128)
+        coef = _np.empty((a.shape[0], *[n + 2 * npad for n in a.shape[1:]]), dtype=_np.complex128)
+        for i in range(a.shape[0]):
+            re = spline_filter(_np.pad(a[i].real, npad, mode="edge"), 3, output=_np.float64, mode="nearest")
+            im = spline_filter(_np.pad(a[i].imag, npad, mode="edge"), 3, output=_np.float64, mode="nearest")
+            coef[i] = re + 1j * im
+        dev = coords.device
+        P = coef.shape[1:]
+        cf = torch.from_numpy(coef.reshape(coef.shape[0], -1)).to(dev, torch.complex128)
+        x = coords.reshape(-1, K).to(torch.float64) + npad
+        start = torch.floor(x).to(torch.int64) - 1
+        y = x - torch.floor(x)
+        w = torch.stack([(1 - y) ** 3 / 6, (3 * y ** 3 - 6 * y ** 2 + 4) / 6,
+                         (-3 * y ** 3 + 3 * y ** 2 + 3 * y + 1) / 6, y ** 3 / 6], dim=-1)  # (npts, K, 4)
+        strides = [math.prod(P[i + 1:]) for i in range(K)]
+        out = torch.zeros((cf.shape[0], x.shape[0]), dtype=torch.complex128, device=dev)
+        for taps in np.ndindex(*(4,) * K):
+            flat = torch.zeros(x.shape[0], dtype=torch.int64, device=dev)
+            wt = torch.ones(x.shape[0], dtype=torch.float64, device=dev)
+            for i, j in enumerate(taps):
+                flat += torch.clamp(start[:, i] + j, 0, P[i] - 1) * strides[i]
+                wt = wt * w[:, i, j]
+            out += cf[:, flat] * wt
+        return out.reshape(cf.shape[0], *coords.shape[:-1]).to(complex_dtype)
+
+    def compute_svd_funcs(self, trj: torch.Tensor, batch_size: Optional[int] = None):
+        """(temporal functions (n_svd, *trj_batch), conj of the spatial functions (n_svd, *im_size))
+        (fourier.py:1413-1470)."""
+        d = len(self.im_size)
+        ss = self.svd_mx_size
+        lams, vecs, mxs = [], [], []
+        for n in ss:
+            r = (np.arange(n) - n // 2) / n
+            m = np.exp(-2j * np.pi * np.outer(r, r))
+            lam, vec = np.linalg.eigh(m.conj().T @ m)
+            lams.append(lam[::-1])
+            vecs.append(vec[:, ::-1])
+            mxs.append(m)
+        keep = [min(n, self.n_svd) for n in ss]
+        cand = sorted(np.ndindex(*keep), key=lambda t: -math.prod(lams[i][t[i]] for i in range(d)))[: self.n_svd]
+        ev = np.stack([_kron_vectors([vecs[i][:, t[i]] for i in range(d)]) for t in cand]).reshape(-1, *ss)
+        tv = np.stack([_kron_vectors([mxs[i] @ vecs[i][:, t[i]] for i in range(d)]) for t in cand]).reshape(-1, *ss)
+        dev = trj.device
+        sz = torch.tensor(ss, dtype=torch.float64, device=dev)
+        spatial = self.spatial_interp(torch.from_numpy(ev), (_gen_grd(self.im_size).to(dev, torch.float64) + 0.5) * sz)
+        trj_dev = trj - (trj * self.grid_oversamp).round() / self.grid_oversamp
+        temporal = self.spatial_interp(torch.from_numpy(tv), (0.5 + trj_dev.to(torch.float64) * self.grid_oversamp) * sz)
+        return temporal, spatial.conj()
+
+    def rescale_trajectory(self, trj: torch.Tensor) -> torch.Tensor:
+        """Basis functions for this trajectory, then round(k s + G/2) mod G (fourier.py:1472-1485)."""
+        temporal, spatial = self.compute_svd_funcs(trj)
+        self.temporal_funcs, self.spatial_funcs = temporal, spatial
+        self.h = temporal.contiguous()
+        self.b = spatial.contiguous()
+        trj_rs = trj * self.grid_oversamp
+        G = self.padder.pad_im_size
+        for i in range(trj_rs.shape[-1]):
+            trj_rs[..., i] = (trj_rs[..., i] + G[i] / 2).round() % G[i]
+        return torch.round(trj_rs).type(torch.int32)
+
+
+def _kron_vectors(vs):
+    out = vs[0]
+    for v in vs[1:]:
+        out = np.multiply.outer(out, v)
+    return out.reshape(-1)
+
+
+def _gen_grd(im_size) -> torch.Tensor:
+    """Pixel grid r_i = (n_i - N_i//2) / N_i, shape (*im_size, d) (utils.py:166-200)."""
+    lins = [torch.arange(-(n // 2), n // 2 + (n % 2)) / n for n in im_size]
+    return torch.stack(torch.meshgrid(*lins, indexing="ij"), dim=-1).to(real_dtype)
+
+
+def _outer_terms(fs):
+    """Products over dims of per-dim terms: fs[i] (n_i, *S) -> (n_0, ..., n_{d-1}, *S)."""
+    out = fs[0]
+    for f in fs[1:]:
+        out = out[:, None] * f[None]
+        out = out.reshape(-1, *f.shape[1:])
+    return out.reshape(*[f.shape[0] for f in fs], *fs[0].shape[1:])
+
+
 __all__ = ["fft", "ifft", "calc_toep_kernel_helper", "NUFFT", "hip_nufft", "triton_nufft", "kb_beta",
-           "sigpy_nufft", "sigpy_beta", "gridded_nufft",
+           "sigpy_nufft", "sigpy_beta", "gridded_nufft", "chebyshev_nufft", "svd_nufft",
            "toeplitz_unshifted", "batch_iterator"]

@@ -1,6 +1,6 @@
 """Linear operators on the NUFFT path: ``linop`` base, ``batching_params``, ``sense_linop``,
-the subspace model (``subspace_linop``) and the type-3 NUFFT (``type3_nufft``,
-``type3_nufft_naive``).
+the low-rank coil / imperfection model (``imperf_coil_lowrank``), the subspace model
+(``subspace_linop``) and the type-3 NUFFT (``type3_nufft``, ``type3_nufft_naive``).
 
 Mirrors the reference (linops.py:22-47, 1041-1308) call for call. With the HIP
 NUFFT and a single imperfection segment (the default), every SENSE
@@ -142,7 +142,16 @@ class type3_nufft(linop):
         sc = (self.grd_os * self.grd_N).ceil() / self.grd_N
         sh = (self.grd_os * self.grd_N).ceil() // 2
         img_trj = (sc * (self.phis.T * self.grd_N / self.grd_gamma) + sh).contiguous()
-        self.img_plan = SamplePlan(img_trj, self.grd_N_os, float(width), [float(b) for b in self.grd_beta.tolist()])
+        betas = [float(b) for b in self.grd_beta.tolist()]
+        if B <= 3:
+            self.img_plan = SamplePlan(img_trj, self.grd_N_os, float(width), betas)
+        else:  # 4 or 5 bases: the direct kernels, coordinates wrapped once here
+            self.img_plan = None
+            G = torch.tensor(self.grd_N_os, dtype=torch.float32, device=img_trj.device)
+            self._img_t = torch.remainder(img_trj.to(torch.float32), G).contiguous()
+            self._img_betas = betas
+            if use_toep:
+                raise NotImplementedError("type3_nufft: Toeplitz kernels with more than 3 bases")
         self._trj = (self.alphas.T * self.grd_gamma).contiguous()          # inner NUFFT trajectory (T, B)
         apod = self.apod(self.alphas.T * self.grd_gamma / self.grd_N / self.grd_os, self.grd_beta,
                          self.grd_W).prod(dim=-1)
@@ -167,12 +176,30 @@ class type3_nufft(linop):
         a = (beta ** 2 - (math.pi * W * x) ** 2) ** 0.5
         return a / torch.sinh(a)
 
+    def _direct(self, adjoint: bool, v: torch.Tensor, n: int) -> torch.Tensor:
+        B, G = self.B, self.grd_N_os
+        P = int(math.floor(self.grd_W)) + 1
+        M = self._img_t.shape[0]
+        args = (B, P, _lib.ptr(v), _lib.ptr(self._img_t), M, n, _lib.ints(G), _lib.floats(self._img_betas),
+                float(self.grd_W), 1.0 / self.grd_W ** B)
+        if adjoint:
+            out = torch.zeros((n, *G), dtype=complex_dtype, device=v.device)
+            _lib.check(_lib.lib().mra_grid_direct(*args, _lib.ptr(out), _lib.stream_ptr(v.device)))
+        else:
+            out = torch.empty((n, M), dtype=complex_dtype, device=v.device)
+            _lib.check(_lib.lib().mra_interp_direct(*args, _lib.ptr(out), _lib.stream_ptr(v.device)))
+        return out
+
     def _grid(self, x: torch.Tensor) -> torch.Tensor:
         n = x.shape[0]
+        if self.img_plan is None:
+            return self._direct(True, x.contiguous(), n)
         out = torch.empty((n, *self.grd_N_os), dtype=complex_dtype, device=x.device)
         return ops.gridding(x, self.img_plan, out, out_scale=1.0 / self.grd_W ** self.B)
 
     def _interp(self, g: torch.Tensor) -> torch.Tensor:
+        if self.img_plan is None:
+            return self._direct(False, g.contiguous(), g.shape[0])
         return ops.interp(g, self.img_plan, out_scale=1.0 / self.grd_W ** self.B)
 
     def forward(self, x: torch.Tensor) -> torch.Tensor:
@@ -217,7 +244,7 @@ class sense_linop(linop):
         dev = trj.device
         assert mps.device == dev
         if nufft is None:
-            nufft = hip_nufft(im_size)
+            nufft = sigpy_nufft(im_size)  # the reference's default (linops.py:1086-1087)
         if dcf is None:
             dcf = torch.ones(trj_size, dtype=real_dtype, device=dev)
         else:
@@ -391,6 +418,100 @@ class sense_linop(linop):
                     f = padder.adjoint(ifft(mf, dim=dims))
                     b = (f * self.spatial_funcs[m1:m2].conj()).sum(dim=-d - 1)
                     out += (b * mps.conj()).sum(dim=0)
+        return out
+
+
+class imperf_coil_lowrank(linop):
+    """Coils and imperfections folded into one low-rank model (linops.py:525-759): the
+    k-space of coil c is sum_l h[l, c] * F(B_l x), spatial functions B (L, *im_size),
+    temporal functions h (L, C, *trj_size).
+
+    One NUFFT call transforms all L images B_l x (the reference loops over segment
+    batches); the contraction with h is one HIP pass (``mra_lowrank_expand``) and the
+    adjoint's conj(h)-weighted, dcf-weighted coil sum another (``mra_lowrank_reduce``).
+    The Toeplitz normal uses the L x L PSF kernels of the weights
+    sum_c conj(h[l1, c]) h[l2, c] dcf (as the reference builds them, :575-590) and returns
+    sum_lo conj(B_lo) crop(iFFT(sum_li K[lo, li] FFT(pad(B_li x)))), i.e. A^H A up to the
+    Toeplitz approximation. (The reference's Toeplitz branch, :738-757, adds the (L, *im)
+    array of un-weighted segment images into a (*im) image, which fails to broadcast.)
+    ``nufft`` defaults to ``sigpy_nufft`` as in the reference.
+    """
+
+    def __init__(self, trj: torch.Tensor, spatial_funcs: torch.Tensor, temporal_funcs: torch.Tensor,
+                 dcf: Optional[torch.Tensor] = None, nufft: Optional[NUFFT] = None,
+                 use_toeplitz: Optional[bool] = False, bparams: Optional[batching_params] = None):
+        im_size = tuple(spatial_funcs.shape[1:])
+        oshape = tuple(temporal_funcs.shape[1:])
+        super().__init__(im_size, oshape)
+        L = spatial_funcs.shape[0]
+        C = temporal_funcs.shape[1]
+        torch_dev = trj.device
+        assert L == temporal_funcs.shape[0]
+        assert temporal_funcs.device == torch_dev
+        assert spatial_funcs.device == torch_dev
+        _lib.require_device(trj, spatial_funcs, temporal_funcs)
+        if nufft is None:
+            nufft = sigpy_nufft(im_size)
+        if dcf is None:
+            dcf = torch.ones(trj.shape[:-1], dtype=real_dtype, device=torch_dev)
+        else:
+            assert dcf.device == torch_dev
+        trj = nufft.rescale_trajectory(trj).type(real_dtype)
+        self.L, self.C = L, C
+        self.im_size = im_size
+        self.trj_size = tuple(trj.shape[:-1])
+        self.use_toeplitz = use_toeplitz
+        self.spatial_funcs = spatial_funcs.type(complex_dtype).contiguous()
+        self.temporal_funcs = temporal_funcs.type(complex_dtype).contiguous()
+        if tuple(self.temporal_funcs.shape[2:]) != self.trj_size:
+            raise ValueError("temporal_funcs must have shape (L, C, *trj_size)")
+        self.trj = trj
+        self.dcf = dcf.type(real_dtype).contiguous()
+        self.nufft = nufft
+        self.bparams = bparams if bparams is not None else batching_params()
+        self.torch_dev = torch_dev
+        self.M = int(math.prod(self.trj_size))
+        if use_toeplitz:
+            h = self.temporal_funcs.reshape(L, C, -1)
+            w = torch.einsum("acm,bcm->abm", h.conj(), h).reshape(L * L, *self.trj_size) * self.dcf
+            kerns = nufft.calc_teoplitz_kernels(trj[None], w)                        # (L L) *im_os
+            self.toep_kerns = kerns.reshape(L, L, *kerns.shape[1:])
+        else:
+            self.toep_kerns = None
+
+    def forward(self, img: torch.Tensor) -> torch.Tensor:
+        """img (*im_size) -> ksp (C, *trj_size)."""
+        bx = img.type(complex_dtype) * self.spatial_funcs                            # L *im
+        fbx = self.nufft.forward(bx[None], self.trj[None])[0].reshape(self.L, self.M).contiguous()
+        ksp = torch.empty((self.C, *self.trj_size), dtype=complex_dtype, device=self.torch_dev)
+        _lib.check(_lib.lib().mra_lowrank_expand(_lib.ptr(fbx), _lib.ptr(self.temporal_funcs), self.L, self.C,
+                                                 self.M, _lib.ptr(ksp), _lib.stream_ptr(self.torch_dev)))
+        return ksp
+
+    def adjoint(self, ksp: torch.Tensor) -> torch.Tensor:
+        """ksp (C, *trj_size) -> img (*im_size), with the dcf applied to ksp first."""
+        y = ops._c64(ksp.type(complex_dtype), "ksp")
+        if y.numel() != self.C * self.M:
+            raise ValueError(f"ksp must have shape ({self.C}, *{self.trj_size})")
+        hdy = torch.empty((self.L, *self.trj_size), dtype=complex_dtype, device=self.torch_dev)
+        _lib.check(_lib.lib().mra_lowrank_reduce(_lib.ptr(y), _lib.ptr(self.temporal_funcs), _lib.ptr(self.dcf),
+                                                 self.L, self.C, self.M, _lib.ptr(hdy),
+                                                 _lib.stream_ptr(self.torch_dev)))
+        f = self.nufft.adjoint(hdy[None], self.trj[None])[0]                          # L *im
+        return (f * self.spatial_funcs.conj()).sum(dim=0)
+
+    def normal(self, img: torch.Tensor) -> torch.Tensor:
+        if not self.use_toeplitz:
+            return self.adjoint(self.forward(img))
+        d = len(self.im_size)
+        im_os = tuple(self.toep_kerns.shape[-d:])
+        padder = PadLast(im_os, self.im_size)
+        dims = tuple(range(-d, 0))
+        F = fft(padder.forward(img.type(complex_dtype) * self.spatial_funcs), dim=dims)   # L *os
+        out = torch.zeros(self.im_size, dtype=complex_dtype, device=img.device)
+        for lo in range(self.L):
+            m = (self.toep_kerns[lo] * F).sum(dim=0)
+            out += padder.adjoint(ifft(m, dim=dims)) * self.spatial_funcs[lo].conj()
         return out
 
 

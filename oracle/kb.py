@@ -64,9 +64,27 @@ def kaiser_bessel_torch(x: torch.Tensor, beta: float) -> torch.Tensor:
 
 
 def kaiser_bessel_torch_safe(x: torch.Tensor, beta: float) -> torch.Tensor:
-    """Same values, but differentiable everywhere: points outside |x| <= 1 are masked
-    before the square root (the reference masks NaNs after it, kernels.py:132, which
-    makes autograd produce 0 * NaN)."""
+    """Same values (the A&S polynomials of kaiser_bessel_torch), written so that autograd is
+    finite everywhere, for finite-difference and gradient checks: points outside |x| <= 1
+    are masked before any square root (the reference masks NaNs after it, kernels.py:132,
+    which makes autograd produce 0 * NaN); the small-argument branch is evaluated from
+    t^2 = beta^2 (1 - x^2) / 3.75^2 directly, a polynomial in x with no square root (its
+    derivative stays finite at |x| = 1, where z = 0); the large-argument branch only sees
+    arguments >= 3.75."""
     inside = x.abs() <= 1
     xs = torch.where(inside, x, torch.zeros_like(x))
-    return torch.where(inside, kaiser_bessel_torch(xs, beta), torch.zeros_like(x))
+    t2 = (beta * beta / (3.75 * 3.75)) * (1 - xs * xs)
+    t4 = t2 * t2
+    t6 = t4 * t2
+    t8 = t4 * t4
+    t10 = t6 * t4
+    t12 = t6 * t6
+    small = (1 + A_SMALL[0] * t2 + A_SMALL[1] * t4 + A_SMALL[2] * t6 + A_SMALL[3] * t8 + A_SMALL[4] * t10
+             + A_SMALL[5] * t12)
+    smallmask = t2 < 1.0
+    z = 3.75 * torch.sqrt(torch.where(smallmask, torch.ones_like(t2), t2))  # >= 3.75 on the branch used
+    t = z / 3.75
+    big = torch.exp(z) / torch.sqrt(z) * (A_BIG[0] + A_BIG[1] / t + A_BIG[2] / t ** 2 + A_BIG[3] / t ** 3
+                                          + A_BIG[4] / t ** 4 + A_BIG[5] / t ** 5 + A_BIG[6] / t ** 6
+                                          + A_BIG[7] / t ** 7 + A_BIG[8] / t ** 8)
+    return torch.where(inside, torch.where(smallmask, small, big), torch.zeros_like(x))

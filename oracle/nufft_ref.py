@@ -285,3 +285,174 @@ def calc_toep_kernel_helper(nufft_adj_os, trj, weights=None, cdtype=torch.comple
     ksp = torch.ones((N, 1, *trj_batch), dtype=cdtype) * weights[:, None, ...]
     img = nufft_adj_os(ksp, trj)[:, 0, ...]
     return fft_c(img, dim=tuple(range(-d, 0)))
+
+
+class ChebyshevNufftRef:
+    """fourier.py:1574-1774 on CPU: Chebyshev expansion of the grid-deviation phase. Basis
+    functions as compute_spatial_funcs / compute_temporal_funcs (:1610-1664) with the Bessel
+    coefficients of cheby_weights (:1601-1608, scipy.special.jv); rescale_trajectory clamps
+    and rounds (:1666-1693); forward / adjoint use PadLast, centred ortho FFTs and
+    multi_index / multi_grid (indexing.py:29-79), times grid_oversamp (:1703-1774)."""
+
+    def __init__(self, im_size, n_cheby_per_dim=5, grid_oversamp=1.0, dtype=torch.float32):
+        self.im_size = tuple(im_size)
+        self.G = tuple(round(i * grid_oversamp) for i in self.im_size)
+        self.grid_oversamp = grid_oversamp
+        self.gamma = 1 / (2 * grid_oversamp)
+        self.n = n_cheby_per_dim
+        self.dtype = dtype
+        self.cdtype = torch.complex64 if dtype == torch.float32 else torch.complex128
+        self.pad = PadLastRef(self.G, self.im_size)
+
+    @staticmethod
+    def cheby_weights(p, r, gamma):
+        from scipy.special import jv
+
+        if (abs(p - r) % 2) == 0:
+            return 4 * (1j ** r) * jv((r + p) / 2, -gamma * np.pi / 2) * jv((r - p) / 2, -gamma * np.pi / 2)
+        return 0
+
+    def spatial_funcs(self):
+        d = len(self.im_size)
+        lins = [(torch.arange(-(n // 2), n // 2 + (n % 2)) / n).to(self.dtype) for n in self.im_size]
+        grd = torch.stack(torch.meshgrid(*lins, indexing="ij"), dim=-1)
+        b = torch.zeros((d, self.n, *self.im_size), dtype=self.cdtype)
+        for i in range(d):
+            r = grd[..., i]
+            for lp in range(self.n):
+                for lq in range(self.n):
+                    s = (0.5 if lp == 0 else 1.0) * (0.5 if lq == 0 else 1.0)
+                    b[i, lp] += torch.cos(lq * torch.arccos(2 * r)) * self.cheby_weights(lp, lq, self.gamma) * s
+        return b
+
+    def temporal_funcs(self, trj):
+        d = trj.shape[-1]
+        dev = trj - torch.round(trj * self.grid_oversamp) / self.grid_oversamp
+        h = torch.zeros((d, self.n, *trj.shape[:-1]), dtype=self.cdtype)
+        for i in range(d):
+            for lp in range(self.n):
+                # clamped to [-1, 1] as the product does (fp32 rounding can leave it an ulp outside)
+                h[i, lp] = torch.cos(lp * torch.arccos(torch.clamp(dev[..., i] / self.gamma, -1.0, 1.0)))
+        return h
+
+    @staticmethod
+    def _outer(fs):
+        out = fs[0]
+        for f in fs[1:]:
+            out = torch.einsum("a...,b...->ab...", out, f).reshape(-1, *f.shape[1:])
+        return out
+
+    def rescale_trajectory(self, trj):
+        t = trj * self.grid_oversamp
+        for i in range(t.shape[-1]):
+            t[..., i] = torch.clamp(t[..., i] + self.G[i] / 2, 0, self.G[i] - 1)
+        t = torch.round(t).type(torch.int32)
+        d = trj.shape[-1]
+        h, b = self.temporal_funcs(trj), self.spatial_funcs()
+        self.h = self._outer([h[i] for i in range(d)]).reshape(-1, *trj.shape[:-1])
+        self.b = self._outer([b[i] for i in range(d)]).reshape(-1, *self.im_size)
+        return t
+
+    def forward(self, img, trj):
+        d = len(self.im_size)
+        xb = img.to(self.cdtype).unsqueeze(-d - 1) * self.b
+        ksp_os = fft_c(self.pad.forward(xb), dim=tuple(range(-d, 0)))
+        outs = []
+        for i in range(trj.shape[0]):
+            t = trj[i].long()
+            ki = ksp_os[i][(Ellipsis,) + tuple(t[..., j] for j in range(d))]  # (*batch, L, *trj)
+            outs.append((ki * self.h).sum(-self.h.ndim))
+        return torch.stack(outs, 0) * self.grid_oversamp
+
+    def adjoint(self, ksp, trj):
+        d = trj.shape[-1]
+        nt = trj.ndim - 2
+        outs = []
+        for i in range(trj.shape[0]):
+            t = trj[i].long().reshape(-1, d)
+            flat = torch.zeros(t.shape[0], dtype=torch.int64)
+            for j in range(d):
+                flat = flat * self.G[j] + t[:, j]
+            y = ksp[i].to(self.cdtype).unsqueeze(-nt - 1) * self.h.conj()  # (*batch, L, *trj)
+            yf = y.reshape(-1, t.shape[0])
+            o = torch.zeros((yf.shape[0], math.prod(self.G)), dtype=self.cdtype)
+            o.index_add_(1, flat, yf)
+            g = o.reshape(*y.shape[:y.ndim - nt], *self.G)
+            img = self.pad.adjoint(ifft_c(g, dim=tuple(range(-d, 0))))
+            outs.append((img * self.b.conj()).sum(-self.b.ndim))
+        return torch.stack(outs, 0) * self.grid_oversamp
+
+
+class SvdNufftRef:
+    """fourier.py:1328-1572 on CPU: low-rank SVD model of the grid-deviation phase. The matrix
+    mx[m, n] = exp(-2 pi i k_m . r_n) on the svd_mx_size pixel grid (k = r, gen_grd), the n_svd
+    largest eigenvectors of mx^H mx (here numpy's eigh in FP64; the reference runs LOBPCG),
+    temporal functions mx @ evecs; both interpolated with cubic splines, mode 'nearest'
+    (scipy.ndimage.map_coordinates, the CPU twin of the reference's cupyx call, :1351-1411) at
+    (gen_grd(im_size) + 1/2) * svd_size and (1/2 + (k - round(k s)/s) s) * svd_size
+    (:1413-1470); rescale_trajectory wraps round(k s + G/2) mod G (:1472-1485);
+    forward / adjoint as :1487-1572."""
+
+    def __init__(self, im_size, grid_oversamp=1.0, n_svd=5, svd_mx_size=None, dtype=torch.float32):
+        self.im_size = tuple(im_size)
+        self.G = tuple(round(i * grid_oversamp) for i in self.im_size)
+        self.grid_oversamp = grid_oversamp
+        self.n_svd = n_svd
+        self.svd_size = (50,) * len(im_size) if svd_mx_size is None else tuple(svd_mx_size)
+        self.cdtype = torch.complex64 if dtype == torch.float32 else torch.complex128
+        self.pad = PadLastRef(self.G, self.im_size)
+
+    @staticmethod
+    def _grd(size):
+        lins = [(np.arange(n) - n // 2) / n for n in size]
+        return np.stack(np.meshgrid(*lins, indexing="ij"), axis=-1)
+
+    @staticmethod
+    def spatial_interp(inp, coords):
+        """inp (N, *dims) complex numpy, coords (..., K) -> (N, ...) (map_coordinates, order 3,
+        mode 'nearest', on the real and imaginary parts)."""
+        from scipy.ndimage import map_coordinates
+
+        flat = coords.reshape(-1, coords.shape[-1]).T
+        out = np.empty((inp.shape[0], flat.shape[1]), dtype=np.complex128)
+        for i in range(inp.shape[0]):
+            re = map_coordinates(inp[i].real, flat, order=3, mode="nearest")
+            im = map_coordinates(inp[i].imag, flat, order=3, mode="nearest")
+            out[i] = re + 1j * im
+        return out.reshape(inp.shape[0], *coords.shape[:-1])
+
+    def svd_funcs(self, trj):
+        ss = self.svd_size
+        rs = self._grd(ss).reshape(-1, len(ss))
+        mx = np.exp(-2j * np.pi * (rs @ rs.T))
+        lam, vec = np.linalg.eigh(mx.conj().T @ mx)
+        order = np.argsort(lam)[::-1][: self.n_svd]
+        evecs = vec[:, order].T.reshape(self.n_svd, *ss)
+        temp = (mx @ vec[:, order]).T.reshape(self.n_svd, *ss)
+        self.eigvals = lam[order]
+        sz = np.array(ss, dtype=np.float64)
+        sp = self.spatial_interp(evecs, (self._grd(self.im_size) + 0.5) * sz)
+        t = trj.detach().cpu().numpy().astype(np.float64)
+        dev = t - np.round(t * self.grid_oversamp) / self.grid_oversamp
+        tp = self.spatial_interp(temp, (0.5 + dev * self.grid_oversamp) * sz)
+        return torch.from_numpy(tp).to(self.cdtype), torch.from_numpy(sp).conj().to(self.cdtype)
+
+    def rescale_trajectory(self, trj):
+        self.h, self.b = self.svd_funcs(trj)
+        t = trj * self.grid_oversamp
+        for i in range(t.shape[-1]):
+            t[..., i] = (t[..., i] + self.G[i] / 2).round() % self.G[i]
+        return torch.round(t).type(torch.int32)
+
+    def forward(self, img, trj):
+        d = len(self.im_size)
+        xb = img.to(self.cdtype).unsqueeze(-d - 1) * self.b
+        ksp_os = fft_c(self.pad.forward(xb), dim=tuple(range(-d, 0)))
+        outs = []
+        for i in range(trj.shape[0]):
+            t = trj[i].long()
+            ki = ksp_os[i][(Ellipsis,) + tuple(t[..., j] for j in range(d))]
+            outs.append((ki * self.h).sum(-self.h.ndim))
+        return torch.stack(outs, 0) * self.grid_oversamp
+
+    adjoint = ChebyshevNufftRef.adjoint

@@ -182,3 +182,98 @@ class SubspaceRef:
                 m = (self.toep_kerns[i] * F).sum(0)
                 out[i] += pad.adjoint(ifft_c(m[None], dims))[0] * self.mps[c].conj()
         return out
+
+
+class SegmentedSenseRef:
+    """``sense_linop`` with L > 1 imperfection segments (linops.py:1134-1308), every batch size
+    1: k-space of coil c is sum_l T_l * F(S_c B_l x); the adjoint weights by the dcf and conj(T_l)
+    and sums conj(S_c) conj(B_l) F^H; the Toeplitz normal mixes the segments through the L x L PSF
+    kernels of conj(T_l1) T_l2 dcf (:1103-1117, 1265-1306)."""
+
+    def __init__(self, trj, mps, spatial_funcs, temporal_funcs, dcf=None, nufft=None, use_toeplitz=False):
+        self.nufft = nufft
+        self.cdt = nufft.cdtype
+        self.im_size = tuple(mps.shape[1:])
+        self.trj_size = tuple(trj.shape[:-1])
+        self.trj = trj.to(torch.float32)
+        self.mps = mps.to(self.cdt)
+        self.B = spatial_funcs.to(self.cdt)
+        self.T = temporal_funcs.to(self.cdt)
+        self.dcf = torch.ones(self.trj_size, dtype=torch.float32) if dcf is None else dcf.to(torch.float32)
+        self.ishape = self.im_size
+        if use_toeplitz:
+            L = self.T.shape[0]
+            kerns = [[nufft.calc_teoplitz_kernels(self.trj[None], (self.T[a].conj() * self.T[b] * self.dcf)[None])[0]
+                      for b in range(L)] for a in range(L)]
+            self.toep_kerns = torch.stack([torch.stack(r, 0) for r in kerns], 0)
+        else:
+            self.toep_kerns = None
+
+    def forward(self, img):
+        ksp = torch.zeros((self.mps.shape[0], *self.trj_size), dtype=self.cdt)
+        for c in range(self.mps.shape[0]):
+            for l in range(self.B.shape[0]):
+                sbx = (self.mps[c] * self.B[l] * img)[None, None]
+                ksp[c] += self.nufft.forward(sbx, self.trj[None])[0, 0] * self.T[l]
+        return ksp
+
+    def adjoint(self, ksp):
+        img = torch.zeros(self.im_size, dtype=self.cdt)
+        for c in range(self.mps.shape[0]):
+            y = ksp[c] * self.dcf
+            for l in range(self.T.shape[0]):
+                f = self.nufft.adjoint((y * self.T[l].conj())[None, None], self.trj[None])[0, 0]
+                img += f * self.mps[c].conj() * self.B[l].conj()
+        return img
+
+    def normal(self, img):
+        if self.toep_kerns is None:
+            return self.adjoint(self.forward(img))
+        d = len(self.im_size)
+        pad = PadLastRef(self.toep_kerns.shape[-d:], self.im_size)
+        dims = tuple(range(-d, 0))
+        L = self.B.shape[0]
+        out = torch.zeros(self.im_size, dtype=self.cdt)
+        for c in range(self.mps.shape[0]):
+            F = [fft_c(pad.forward((self.mps[c] * self.B[l] * img)[None]), dims)[0] for l in range(L)]
+            for m in range(L):
+                acc = sum(self.toep_kerns[m, l] * F[l] for l in range(L))
+                out += pad.adjoint(ifft_c(acc[None], dims))[0] * self.B[m].conj() * self.mps[c].conj()
+        return out
+
+
+class ImperfCoilLowrankRef:
+    """``imperf_coil_lowrank`` (linops.py:525-759), batch sizes 1: k-space of coil c is
+    sum_l h[l, c] * F(B_l x) (:614-652); the adjoint sums conj(B_l) F^H(sum_c conj(h[l, c]) dcf
+    y_c) (:654-714). The normal is adjoint(forward) (the reference's Toeplitz branch does not
+    run, see the product's docstring)."""
+
+    def __init__(self, trj, spatial_funcs, temporal_funcs, dcf=None, nufft=None):
+        self.nufft = nufft
+        self.cdt = nufft.cdtype
+        self.B = spatial_funcs.to(self.cdt)
+        self.h = temporal_funcs.to(self.cdt)
+        self.trj = trj.to(torch.float32)
+        self.trj_size = tuple(trj.shape[:-1])
+        self.im_size = tuple(spatial_funcs.shape[1:])
+        self.dcf = torch.ones(self.trj_size, dtype=torch.float32) if dcf is None else dcf.to(torch.float32)
+
+    def forward(self, img):
+        L, C = self.h.shape[:2]
+        ksp = torch.zeros((C, *self.trj_size), dtype=self.cdt)
+        for l in range(L):
+            f = self.nufft.forward((img * self.B[l])[None, None], self.trj[None])[0, 0]
+            for c in range(C):
+                ksp[c] += f * self.h[l, c]
+        return ksp
+
+    def adjoint(self, ksp):
+        L, C = self.h.shape[:2]
+        img = torch.zeros(self.im_size, dtype=self.cdt)
+        for l in range(L):
+            hdy = sum(ksp[c] * self.dcf * self.h[l, c].conj() for c in range(C))
+            img += self.nufft.adjoint(hdy[None, None], self.trj[None])[0, 0] * self.B[l].conj()
+        return img
+
+    def normal(self, img):
+        return self.adjoint(self.forward(img))

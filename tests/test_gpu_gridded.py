@@ -56,3 +56,30 @@ def test_gridded_exact_on_lattice_and_toeplitz(dev):
     K = hn.calc_teoplitz_kernels(t, w.to(dev))
     ref = calc_toep_kernel_helper(GriddedNufftRef(hn.im_size_os, 1.0).adjoint, t.cpu(), w) * os_ ** 2
     assert rel_l2(K, ref) < TOL
+
+
+@pytest.mark.parametrize("im,os_,nspokes", [((64, 64), 2.0, 400), ((16, 16, 16), 1.5, 400)])
+def test_gridded_long_runs_vs_oracle(dev, im, os_, nspokes):
+    """Radial spokes all cross the k-space centre, so the centre point collects far more than
+    one scatter chunk (64 samples) -- the multi-chunk split of the fused adjoint runs -- and
+    the adjoint, called twice on one trajectory, reuses its cached index tables."""
+    from mr_recon_amd.fourier import gridded_nufft
+
+    d = len(im)
+    n = im[0]
+    k = (synth.radial_2d(nspokes, 2 * n, n) if d == 2 else synth.kooshball_3d(nspokes, 2 * n, n))[None]
+    hn, rn = gridded_nufft(im, os_), GriddedNufftRef(im, os_)
+    t = hn.rescale_trajectory(k.clone().to(dev))
+    tr = rn.rescale_trajectory(k.clone())
+    assert torch.equal(t.cpu(), tr)
+    idx = hn._flat_index(t[0], hn.im_size_os, unshifted=True)
+    runs = torch.unique(idx, return_counts=True)[1]
+    assert int(runs.max()) > 4 * hn.CHUNK
+    x = synth.random_image((1, 2, *im), seed=3)
+    y = hn.forward(x.to(dev), t)
+    assert rel_l2(y, rn.forward(x, tr)) < TOL
+    yy = synth.random_image(tuple(y.shape), seed=5)
+    z_ref = rn.adjoint(yy, tr)
+    for _ in range(2):
+        z = hn.adjoint(yy.to(dev), t)
+        assert rel_l2(z, z_ref) < TOL

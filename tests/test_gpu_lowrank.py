@@ -1,0 +1,75 @@
+"""GPU parity of the segmented models: sense_linop with L > 1 spatial / temporal imperfection
+functions (linops.py:1134-1308, incl. the L x L Toeplitz normal) and imperf_coil_lowrank
+(linops.py:525-759) against the oracle."""
+
+import pytest
+import torch
+
+from helpers import cdot, rel_l2
+from mr_recon_amd import synth
+from oracle.nufft_ref import SigpyNufftRef, TritonNufftRef
+from oracle.sense_ref import ImperfCoilLowrankRef, SegmentedSenseRef
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _funcs(L, im, trj_size, seed):
+    g = torch.Generator().manual_seed(seed)
+    B = torch.complex(torch.randn((L, *im), generator=g), torch.randn((L, *im), generator=g)) * 0.7
+    T = torch.complex(torch.randn((L, *trj_size), generator=g), torch.randn((L, *trj_size), generator=g)) * 0.7
+    return B, T
+
+
+@pytest.mark.parametrize("im,toep", [((32, 32), False), ((32, 32), True), ((12, 12, 12), False)])
+def test_sense_segments_vs_oracle(dev, im, toep):
+    from mr_recon_amd.fourier import hip_nufft
+    from mr_recon_amd.linops import sense_linop
+
+    d = len(im)
+    trj = synth.spiral_2d(6, 300, im[0]) if d == 2 else synth.kooshball_3d(60, 24, im[0])
+    mps = synth.coil_maps(3, im, seed=1)
+    B, T = _funcs(3, im, trj.shape[:-1], seed=2)
+    dcf = torch.rand(trj.shape[:-1], generator=torch.Generator().manual_seed(3)) + 0.5
+    A = sense_linop(trj.to(dev), mps.to(dev), dcf=dcf.to(dev), nufft=hip_nufft(im, 1.25, 4), use_toeplitz=toep,
+                    spatial_funcs=B.to(dev), temporal_funcs=T.to(dev))
+    R = SegmentedSenseRef(trj, mps, B, T, dcf=dcf, nufft=TritonNufftRef(im, 1.25, 4), use_toeplitz=toep)
+    x = synth.random_image(im, seed=4)
+    y = A.forward(x.to(dev))
+    assert rel_l2(y, R.forward(x)) < TOL
+    k = synth.random_image(tuple(y.shape), seed=5)
+    z = A.adjoint(k.to(dev))
+    assert rel_l2(z, R.adjoint(k)) < TOL
+    assert abs(cdot(k * dcf, y) - cdot(z, x)) < 1e-5 * abs(cdot(k * dcf, y))
+    if toep:
+        assert rel_l2(A.toep_kerns, R.toep_kerns) < TOL
+        assert rel_l2(A.normal(x.to(dev)), R.normal(x)) < TOL
+
+
+@pytest.mark.parametrize("im,toep", [((40, 40), False), ((40, 40), True), ((12, 14, 10), False)])
+def test_imperf_coil_lowrank_vs_oracle(dev, im, toep):
+    from mr_recon_amd.linops import imperf_coil_lowrank
+
+    d = len(im)
+    trj = synth.radial_2d(30, 80, im[0]) if d == 2 else (torch.rand((900, 3)) - 0.5) * torch.tensor(im)
+    L, C = 3, 4
+    g = torch.Generator().manual_seed(7)
+    B = torch.complex(torch.randn((L, *im), generator=g), torch.randn((L, *im), generator=g))
+    h = torch.complex(torch.randn((L, C, *trj.shape[:-1]), generator=g), torch.randn((L, C, *trj.shape[:-1]),
+                                                                                      generator=g))
+    dcf = torch.rand(trj.shape[:-1], generator=g) + 0.5
+    A = imperf_coil_lowrank(trj.to(dev), B.to(dev), h.to(dev), dcf=dcf.to(dev), use_toeplitz=toep)
+    R = ImperfCoilLowrankRef(trj, B, h, dcf=dcf, nufft=SigpyNufftRef(im))
+    x = synth.random_image(im, seed=8)
+    y = A.forward(x.to(dev))
+    assert y.shape == (C, *trj.shape[:-1])
+    assert rel_l2(y, R.forward(x)) < TOL
+    k = synth.random_image(tuple(y.shape), seed=9)
+    z = A.adjoint(k.to(dev))
+    assert rel_l2(z, R.adjoint(k)) < TOL
+    assert abs(cdot(k * dcf, y) - cdot(z, x)) < 1e-5 * abs(cdot(k * dcf, y))
+    if toep:
+        # Toeplitz normal vs adjoint(forward): the Kaiser-Bessel approximation error of (6, 1.25)
+        n0 = R.normal(x)
+        assert rel_l2(A.normal(x.to(dev)), n0) < 5e-3

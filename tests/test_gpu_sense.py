@@ -88,6 +88,40 @@ def test_cg_sense_vs_oracle(dev, toeplitz):
     assert rel_l2(got, ref64) < max(1e-5, 5 * e32)
 
 
+@pytest.mark.parametrize("toeplitz", [False, True])
+def test_cg_sense_default_power_method_vs_oracle(dev, monkeypatch, toeplitz):
+    """The reference's default CG-SENSE branch, max_eigen=None (recons.py:119-123): 15 power
+    iterations (algs.py:364-399) from torch.randn, then CG on the 1.01 lambda_max-scaled system.
+    Both sides draw the same start vector (torch.randn is replaced by one fixed draw)."""
+    from mr_recon_amd.algs import power_method_operator
+    from mr_recon_amd.fourier import hip_nufft
+    from mr_recon_amd.linops import sense_linop
+    from mr_recon_amd.recons import CG_SENSE_recon
+    from oracle.sense_ref import power_method_operator as pm_ref
+
+    im = (48, 48)
+    trj = synth.spiral_2d(12, 400, 48)
+    mps = synth.coil_maps(6, im, seed=4)
+    x = synth.phantom_2d(48, seed=1)
+    x0 = synth.random_image(im, seed=21)
+    R64 = SenseRef(trj, mps.to(torch.complex128), nufft=TritonNufftRef(im, 1.25, 4, dtype=torch.float64),
+                   use_toeplitz=toeplitz)
+    R32 = SenseRef(trj, mps, nufft=TritonNufftRef(im, 1.25, 4), use_toeplitz=toeplitz)
+    y = R64.forward(x.to(torch.complex128))
+    A = sense_linop(trj.to(dev), mps.to(dev), nufft=hip_nufft(im, 1.25, 4), use_toeplitz=toeplitz)
+
+    _, lam = power_method_operator(A.normal, x0.to(dev), verbose=False)
+    _, lam64 = pm_ref(R64.normal, x0.to(torch.complex128))
+    assert abs(lam - lam64) < 1e-5 * lam64
+
+    monkeypatch.setattr(torch, "randn", lambda *a, dtype=None, device=None, **k: x0.to(dtype).to(device))
+    kw = dict(max_iter=10, lamda_l2=1e-3, max_eigen=None, tolerance=0.0)
+    got = CG_SENSE_recon(A, y.to(torch.complex64).to(dev), verbose=False, **kw)
+    ref64 = cg_sense_recon(R64, y, **kw)
+    ref32 = cg_sense_recon(R32, y.to(torch.complex64), **kw)
+    assert rel_l2(got, ref64) < max(1e-5, 5 * rel_l2(ref32, ref64))
+
+
 def test_cg_small_spd(dev):
     """Fused CG kernels on a dense SPD operator vs torch.linalg.solve and the oracle CG."""
     from mr_recon_amd.algs import conjugate_gradient as cg
@@ -105,3 +139,20 @@ def test_cg_small_spd(dev):
     assert rel_l2(x5, ref) < 1e-5
     resids, xs = cg(lambda v: Hd @ v, b.to(torch.complex64).to(dev), num_iters=5, return_resids=True)
     assert len(resids) == 5 and len(xs) == 6
+
+
+def test_sense_default_nufft_is_sigpy_nufft(dev):
+    """Without a nufft argument sense_linop uses sigpy_nufft(im_size), as the reference does
+    (linops.py:1086-1087), and matches the oracle of that transform."""
+    from mr_recon_amd.fourier import sigpy_nufft
+    from mr_recon_amd.linops import sense_linop
+    from oracle.nufft_ref import SigpyNufftRef
+
+    im = (32, 32)
+    trj = synth.spiral_2d(6, 300, 32)
+    mps = synth.coil_maps(4, im, seed=6)
+    A = sense_linop(trj.to(dev), mps.to(dev))
+    assert type(A.nufft) is sigpy_nufft and A.nufft.width == 6 and A.nufft.oversamp == 1.25
+    R = SenseRef(trj, mps, nufft=SigpyNufftRef(im))
+    x = synth.random_image(im, seed=7)
+    assert rel_l2(A.forward(x.to(dev)), R.forward(x)) < TOL

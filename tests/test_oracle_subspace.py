@@ -49,3 +49,39 @@ def test_subspace_segments_and_toeplitz():
     assert abs(cdot(ksp, y) - cdot(A.adjoint(ksp), alphas)) < 1e-10 * abs(cdot(ksp, y))
     T = SubspaceRef(trj, mps, phi, nufft=nf, use_toeplitz=True)
     assert rel_l2(T.normal(alphas), T.adjoint(T.forward(alphas))) < 1e-3
+
+
+def test_segmented_sense_and_lowrank_oracles():
+    """SegmentedSenseRef with one segment of ones == SenseRef; several segments == the sum over
+    segments of SenseRef applied to B_l x, weighted by T_l; ImperfCoilLowrankRef with B_l = S_l and
+    h[l, c] = delta_lc == SenseRef; both adjoint to their forward (FP64)."""
+    import torch
+
+    from helpers import cdot, rel_l2
+    from mr_recon_amd import synth
+    from oracle.nufft_ref import TritonNufftRef
+    from oracle.sense_ref import ImperfCoilLowrankRef, SegmentedSenseRef, SenseRef
+
+    im = (16, 16)
+    trj = synth.spiral_2d(4, 60, 16)
+    mps = synth.coil_maps(3, im, seed=1).to(torch.complex128)
+    nf = TritonNufftRef(im, 2.0, 6, dtype=torch.float64)
+    x = synth.random_image(im, seed=2).to(torch.complex128)
+    g = torch.Generator().manual_seed(0)
+    B = torch.randn((2, *im), dtype=torch.complex128, generator=g)
+    T = torch.randn((2, *trj.shape[:-1]), dtype=torch.complex128, generator=g)
+    dcf = torch.rand(trj.shape[:-1], generator=g) + 0.5
+    R = SenseRef(trj, mps, dcf=dcf, nufft=nf)
+    one = SegmentedSenseRef(trj, mps, torch.ones((1, *im)), torch.ones((1, *trj.shape[:-1])), dcf=dcf, nufft=nf)
+    assert rel_l2(one.forward(x), R.forward(x)) < 1e-12
+    S = SegmentedSenseRef(trj, mps, B, T, dcf=dcf, nufft=nf)
+    want = sum(R.forward(B[l] * x) * T[l] for l in range(2))
+    assert rel_l2(S.forward(x), want) < 1e-12
+    y = synth.random_image((3, *trj.shape[:-1]), seed=3).to(torch.complex128)
+    assert abs(cdot(y * dcf, S.forward(x)) - cdot(S.adjoint(y), x)) < 1e-10 * abs(cdot(y * dcf, S.forward(x)))
+    h = torch.eye(3, dtype=torch.complex128)[:, :, None, None].expand(3, 3, *trj.shape[:-1])
+    LR = ImperfCoilLowrankRef(trj, mps, h, dcf=dcf, nufft=nf)
+    assert rel_l2(LR.forward(x), R.forward(x)) < 1e-12
+    assert rel_l2(LR.adjoint(y), R.adjoint(y)) < 1e-12
+    St = SegmentedSenseRef(trj, mps, B, T, dcf=dcf, nufft=nf, use_toeplitz=True)
+    assert rel_l2(St.normal(x), S.normal(x)) < 5e-3
