@@ -54,6 +54,12 @@ template <> struct TileCfg<3> { static constexpr int T = 8; };
 #define MRA_INTERP_LDS_KB 32
 #endif
 constexpr int kLdsBudget = MRA_INTERP_LDS_KB * 1024;  // bytes of LDS subgrids per interp block (more, smaller blocks hide latency better)
+// Waves per SIMD that the LDS budget leaves room for (160 KiB LDS per CU, 4 SIMDs). Telling
+// the compiler this bound (amdgpu_waves_per_eu) lets it spend the registers that the
+// unreachable occupancies would have needed on more LDS reads in flight: at 3D P = 5 the
+// inner loop issues 8 128-bit reads before its first wait instead of 2.
+constexpr int kInterpWavesPerEU = (160 * 1024 / kLdsBudget) * (kThreads / 64) / 4;
+static_assert(kInterpWavesPerEU >= 1 && kInterpWavesPerEU <= 8, "interp LDS budget out of range");
 
 // Offset in the subgrid of patch point q of the trailing D-1 dims.
 template <int D, int P, int T>
@@ -232,7 +238,7 @@ __global__ __launch_bounds__(kThreads) void interp_kernel(const float2* __restri
 // 128-bit reads than with 64-bit ones). Same work decomposition as interp_kernel; an odd
 // coil count pads its last pair with zeros.
 template <int D, int P, int T>
-__global__ __launch_bounds__(kThreads) void interp_pair_kernel(const float2* __restrict__ grid,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, kInterpWavesPerEU))) void interp_pair_kernel(const float2* __restrict__ grid,
                                                                float2* __restrict__ out,
                                                                const float* __restrict__ wsort,
                                                                const int* __restrict__ lbase,
