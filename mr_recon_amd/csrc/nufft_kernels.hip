@@ -105,15 +105,22 @@ __device__ __forceinline__ void subgrid_offsets(const Geom& g, const int (&tc)[D
   }
 }
 
+// Returns whether the last patch point of some dimension carries weight. A point is in the
+// support when |g - t| <= w/2, so a sample reaches P = floor(w) + 1 points along a dimension
+// only where t - w/2 falls on a grid integer (for an integer width; the k-space centre of a
+// radial or kooshball trajectory, say); everywhere else that point's weight is exactly 0.
 template <int D, int P>
-__device__ __forceinline__ void load_weights(const float* __restrict__ wsort, long long s, float scale,
+__device__ __forceinline__ bool load_weights(const float* __restrict__ wsort, long long s, float scale,
                                              float (&wx)[P], float (&wr)[ipow(P, D - 1)]) {
   const float* w = wsort + s * (D * P);
   float wd[D][P];
+  bool edge = false;
 #pragma unroll
-  for (int d = 0; d < D; ++d)
+  for (int d = 0; d < D; ++d) {
 #pragma unroll
     for (int j = 0; j < P; ++j) wd[d][j] = w[d * P + j];
+    edge = edge || wd[d][P - 1] != 0.0f;
+  }
 #pragma unroll
   for (int i = 0; i < P; ++i) wx[i] = wd[0][i] * scale;
   if constexpr (D == 3) {
@@ -127,6 +134,73 @@ __device__ __forceinline__ void load_weights(const float* __restrict__ wsort, lo
   } else {
     wr[0] = 1.0f;
   }
+  return edge;
+}
+
+// Index in wr (patch points of dims 1..D-1, P per dim) of point u of the leading PE x PE
+// block (PE <= P points per dim).
+template <int D, int P, int PE>
+__device__ __forceinline__ constexpr int sub_q(int u) {
+  if constexpr (D == 3) return (u / PE) * P + (u % PE);
+  else if constexpr (D == 2) return u;
+  else return 0;
+}
+
+// sum over the first PE patch points of every dimension of w * v (one coil, complex).
+// With PE = P - 1 and zero weights on the last points this is the full sum bit for bit:
+// each skipped term is a fused multiply-add of 0 into the running sum.
+template <int D, int P, int T, int PE>
+__device__ __forceinline__ float2 patch_sum(const float2* sb, const float (&wx)[P],
+                                            const float (&wr)[ipow(P, D - 1)]) {
+  using TS = TileShape<D, P, T>;
+  constexpr int NR = ipow(PE, D - 1);
+  constexpr int UI = NR <= 36 ? PE : 1;  // unroll dim 0 unless the registers would cap occupancy
+  float ax = 0.f, ay = 0.f;
+#pragma unroll UI
+  for (int i = 0; i < PE; ++i) {
+    float bx = 0.f, by = 0.f;
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+      const int q = sub_q<D, P, PE>(u);
+      const float2 v = sb[i * TS::SUBD1 + rest_off<D, P, T>(q)];
+      bx += wr[q] * v.x;
+      by += wr[q] * v.y;
+    }
+    ax += wx[i] * bx;
+    ay += wx[i] * by;
+  }
+  return make_float2(ax, ay);
+}
+
+// Same for a coil pair stored interleaved (float4 = two coils' values of a point).
+template <int D, int P, int T, int PE>
+__device__ __forceinline__ float4 patch_sum_pair(const float4* sb, const float (&wx)[P],
+                                                 const float (&wr)[ipow(P, D - 1)]) {
+  using TS = TileShape<D, P, T>;
+  constexpr int NR = ipow(PE, D - 1);
+  float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
+#ifdef MRA_INTERP_PAIR_UNROLL
+#pragma unroll
+#else
+#pragma unroll 1
+#endif
+  for (int i = 0; i < PE; ++i) {
+    float bx = 0.f, by = 0.f, bz = 0.f, bw = 0.f;
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+      const int q = sub_q<D, P, PE>(u);
+      const float4 v = sb[i * TS::SUBD1 + rest_off<D, P, T>(q)];
+      bx += wr[q] * v.x;
+      by += wr[q] * v.y;
+      bz += wr[q] * v.z;
+      bw += wr[q] * v.w;
+    }
+    ax += wx[i] * bx;
+    ay += wx[i] * by;
+    az += wx[i] * bz;
+    aw += wx[i] * bw;
+  }
+  return make_float4(ax, ay, az, aw);
 }
 
 // --------------------------------------------------------------------------------------------
@@ -208,25 +282,14 @@ __global__ __launch_bounds__(kThreads) void interp_kernel(const float2* __restri
   for (int s = it.z + tid; s < it.z + it.w; s += kThreads) {
     const long long gs = g.samp_off + s;
     float wx[P], wr[TS::PR];
-    load_weights<D, P>(wsort, gs, g.out_scale, wx, wr);
+    const bool edge = load_weights<D, P>(wsort, gs, g.out_scale, wx, wr);
+    const bool full = __any(edge);  // wave-uniform: P points per dim only where a lane needs them
     const int base = lbase[gs];
     const int o = sample_slot(perm, g, gs);
     for (int cl = 0; cl < nc; ++cl) {
       const float2* sb = lds2 + cl * TS::SUBN + base;
-      float ax = 0.f, ay = 0.f;
-#pragma unroll
-      for (int i = 0; i < P; ++i) {
-        float bx = 0.f, by = 0.f;
-#pragma unroll
-        for (int q = 0; q < TS::PR; ++q) {
-          const float2 v = sb[i * TS::SUBD1 + rest_off<D, P, T>(q)];
-          bx += wr[q] * v.x;
-          by += wr[q] * v.y;
-        }
-        ax += wx[i] * bx;
-        ay += wx[i] * by;
-      }
-      odst[(long long)cl * g.ksp_stride + o] = make_float2(ax, ay);
+      odst[(long long)cl * g.ksp_stride + o] =
+          full ? patch_sum<D, P, T, P>(sb, wx, wr) : patch_sum<D, P, T, P - 1>(sb, wx, wr);
     }
   }
 }
@@ -291,34 +354,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, kIn
   for (int s = it.z + tid; s < it.z + it.w; s += kThreads) {
     const long long gs = g.samp_off + s;
     float wx[P], wr[TS::PR];
-    load_weights<D, P>(wsort, gs, g.out_scale, wx, wr);
+    const bool edge = load_weights<D, P>(wsort, gs, g.out_scale, wx, wr);
+    const bool full = __any(edge);  // wave-uniform: P points per dim only where a lane needs them
     const int base = lbase[gs];
     const int o = sample_slot(perm, g, gs);
     for (int cp = 0; cp < npair; ++cp) {
       const float4* sb = lds4 + cp * TS::SUBN + base;
-      float ax = 0.f, ay = 0.f, az = 0.f, aw = 0.f;
-#ifdef MRA_INTERP_PAIR_UNROLL
-#pragma unroll
-#else
-#pragma unroll 1
-#endif
-      for (int i = 0; i < P; ++i) {
-        float bx = 0.f, by = 0.f, bz = 0.f, bw = 0.f;
-#pragma unroll
-        for (int q = 0; q < TS::PR; ++q) {
-          const float4 v = sb[i * TS::SUBD1 + rest_off<D, P, T>(q)];
-          bx += wr[q] * v.x;
-          by += wr[q] * v.y;
-          bz += wr[q] * v.z;
-          bw += wr[q] * v.w;
-        }
-        ax += wx[i] * bx;
-        ay += wx[i] * by;
-        az += wx[i] * bz;
-        aw += wx[i] * bw;
-      }
-      odst[(long long)(2 * cp) * g.ksp_stride + o] = make_float2(ax, ay);
-      if (2 * cp + 1 < nc) odst[(long long)(2 * cp + 1) * g.ksp_stride + o] = make_float2(az, aw);
+      const float4 a = full ? patch_sum_pair<D, P, T, P>(sb, wx, wr) : patch_sum_pair<D, P, T, P - 1>(sb, wx, wr);
+      odst[(long long)(2 * cp) * g.ksp_stride + o] = make_float2(a.x, a.y);
+      if (2 * cp + 1 < nc) odst[(long long)(2 * cp + 1) * g.ksp_stride + o] = make_float2(a.z, a.w);
     }
   }
 }

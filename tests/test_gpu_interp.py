@@ -80,6 +80,34 @@ def test_ungrid_matches_oracle(dev, D, G, width, B):
     assert rel_l2(out, ref) < TOL
 
 
+@pytest.mark.parametrize("D,G,width,B", [(2, 48, 4.0, 2), (2, 40, 5.0, 3), (3, 20, 4.0, 2), (3, 20, 4.0, 3),
+                                         (3, 16, 6.0, 2), (1, 64, 4.0, 1)])
+@pytest.mark.parametrize("frac", [0.0, 0.01, 1.0])
+def test_ungrid_full_and_short_patches(dev, D, G, width, B, frac):
+    """A sample reaches floor(w) + 1 points along a dimension only where t - w/2 is a grid
+    integer; elsewhere the last patch point has weight 0 and the gather sums one point
+    fewer per dimension. Waves with no such sample, some, and only such samples (a
+    fraction ``frac`` of the samples on the lattice, in one or all dimensions), coil
+    chunks of odd and even size, against the oracle."""
+    from mr_recon_amd.hip_interp import ungrid
+
+    g = torch.Generator().manual_seed(D * 1000 + G + int(10 * width) + int(100 * frac))
+    n = 3000
+    k = torch.rand((n, D), generator=g) * G
+    on = torch.rand(n, generator=g) < frac
+    lattice = torch.floor(k) + (width / 2 - math.floor(width / 2))
+    alldims = torch.rand(n, generator=g) < 0.5
+    dim0 = torch.zeros_like(k, dtype=torch.bool)
+    dim0[:, 0] = True
+    sel = on[:, None] & (alldims[:, None] | dim0)
+    k = torch.where(sel, lattice, k)
+    vals = torch.randn((B, *(G,) * D), dtype=torch.complex64, generator=g)
+    beta = 6.7
+    out = ungrid(vals.to(dev), k.to(dev), width, kernel_params={"beta": beta})
+    ref = interp_ref.ungrid(vals, k, width, beta=beta)
+    assert rel_l2(out, ref) < TOL
+
+
 @pytest.mark.parametrize("D,G,width,B", [(1, 128, 4.0, 2), (2, 48, 4.0, 2), (2, 64, 6.0, 3), (3, 20, 4.0, 2),
                                          (3, 16, 6.0, 1), (2, 32, 3.0, 1)])
 def test_grid_matches_oracle(dev, D, G, width, B):
