@@ -247,10 +247,6 @@ def run_grad(args):
     print(json.dumps(measure_grad(args.steps, args.warmup)), flush=True)
 
 
-def _dot_re64(a, b):
-    return torch.real(torch.sum(a.conj() * b, dtype=torch.complex128))
-
-
 def measure_grad(steps, warmup):
     """loss = ||A(k) x - y||^2 + ||CG_10(A(k), y) - x0||^2, 256^2 image, 16 coils, 64 spiral
     interleaves x 2048; gradients w.r.t. x and the trajectory k. Both paths reduce the CG
@@ -269,18 +265,18 @@ def measure_grad(steps, warmup):
         y = y + 0.01 * synth.random_image(y.shape, seed=9).to(dev) * y.abs().mean()
     xin = (x0 * 0.9).detach()
     kin = (k0 + 0.05).detach()
-    lam, eig, iters = 1e-3, 1.0, 10
+    lam, eig, iters = 0.05, 2204.0, 10  # eig: 1.01 x lambda_max of A^H A (FP64 power method)
 
-    def fused():
+    def fused(n_it=iters):
         x = xin.clone().requires_grad_(True)
         k = kin.clone().requires_grad_(True)
         dc = torch.sum(torch.abs(diff.sense_forward(x, k, mps, nf) - y) ** 2)
-        rec = diff.unrolled_cg_sense(y, k, mps, nf, num_iters=iters, lamda_l2=lam, max_eigen=eig, checkpoint=True)
+        rec = diff.unrolled_cg_sense(y, k, mps, nf, num_iters=n_it, lamda_l2=lam, max_eigen=eig, checkpoint=True)
         loss = dc + torch.sum(torch.abs(rec - x0) ** 2)
         loss.backward()
         return loss.detach(), x.grad, k.grad
 
-    def plain():
+    def plain(n_it=iters):
         """Per-coil differentiable NUFFTs composed in Python, unrolled CG without checkpointing."""
         x = xin.clone().requires_grad_(True)
         k = kin.clone().requires_grad_(True)
@@ -297,13 +293,13 @@ def measure_grad(steps, warmup):
         xx = ahb
         r = ahb - (AH(A(xx)) * inv + lam * xx)
         p = r
-        rz = _dot_re64(r, r)
-        for _ in range(iters):
+        rz = diff.dot_re64(r, r)
+        for _ in range(n_it):
             ap = AH(A(p)) * inv + lam * p
-            a = (rz / _dot_re64(p, ap)).to(torch.float32)
+            a = (rz / diff.dot_re64(p, ap)).to(torch.float32)
             xx = xx + a * p
             r = r - a * ap
-            rzn = _dot_re64(r, r)
+            rzn = diff.dot_re64(r, r)
             p = r + (rzn / rz).to(torch.float32) * p
             rz = rzn
         loss = dc + torch.sum(torch.abs(xx - x0) ** 2)
@@ -332,6 +328,10 @@ def measure_grad(steps, warmup):
 
     ldf, gxdf, gkdf = dc_only(True)
     ldp, gxdp, gkdp = dc_only(False)
+    # three CG iterations: fp32 CG still tracks FP64 CG there (with ten it departs near
+    # convergence, and dL/dk of the fp32 oracle itself is 1.5 relative L2 from FP64)
+    l3f, _, gk3f = fused(3)
+    l3p, _, gk3p = plain(3)
 
     rec = {
         "metric": "loss+gradient evaluations/s (2D 256^2 x 16 coils SENSE NUFFT + 10-iteration unrolled CG)",
@@ -346,6 +346,7 @@ def measure_grad(steps, warmup):
         "speedup_vs_plain": t_p / t_f,
         "agreement": {"loss": abs(float(lf - lp)) / abs(float(lp)), "grad_x": rel(gxf, gxp),
                       "grad_k": rel(gkf, gkp),
+                      "cg_iters_3": {"loss": abs(float(l3f - l3p)) / abs(float(l3p)), "grad_k": rel(gk3f, gk3p)},
                       "data_term_only": {"loss": abs(float(ldf - ldp)) / abs(float(ldp)), "grad_x": rel(gxdf, gxdp),
                                          "grad_k": rel(gkdf, gkdp)}},
     }
@@ -466,7 +467,9 @@ def run_rows(args):
     its oracle on a bounded CPU sample (scaled linearly by samples):
     sigpy_nufft (w 6, sigma 1.25) and gridded_nufft (sigma 2) on the 3D kooshball with 32
     coils; subspace_linop 2D 256^2 x 16 coils x 5 coefficients on 32 spiral interleaves x
-    1024 x 8 time points; type3_nufft 2D 128^2 at 64 x 1024 samples (w 4, sigma 2)."""
+    1024 x 8 time points; type3_nufft 2D 128^2 at 64 x 1024 samples (w 4, sigma 2);
+    chebyshev_nufft and svd_nufft (2D 256^2, 16 images), sense_linop with 3 imperfection
+    segments and imperf_coil_lowrank (rank 4, 16 coils) on the 32 x 4096 spiral."""
     import numpy as np
 
     from mr_recon_amd import synth
@@ -579,6 +582,46 @@ def run_rows(args):
         cs = cpu_time(lambda: R3.adjoint(R3.forward(x3)))
     emit("type3_nufft 2D 128^2, 65536 samples", {"width": 4.0, "oversamp": 2.0}, t, cs,
          "the whole workload through the oracle")
+    del T3
+
+    # low-rank nearest-grid NUFFTs and the segmented / low-rank SENSE models on the 2D
+    # spiral of the CG-SENSE workload (256^2, 32 x 4096 samples), 16 coil images per call
+    from mr_recon_amd.fourier import chebyshev_nufft, svd_nufft
+    from mr_recon_amd.linops import imperf_coil_lowrank, sense_linop
+
+    n4 = 256
+    sp2 = synth.spiral_2d(32, 4096, n4).reshape(-1, 2)
+    x16 = synth.random_image((1, 16, n4, n4), seed=3).to(dev)
+    for name, nf, cfg in (("chebyshev_nufft", chebyshev_nufft((n4, n4), 5, 1.25), {"n_cheby_per_dim": 5}),
+                          ("svd_nufft", svd_nufft((n4, n4), 1.25, 6), {"n_svd": 6})):
+        ti = nf.rescale_trajectory(sp2.clone().to(dev))[None]
+        t = timed_pair(lambda x: nf.forward(x, ti), lambda y: nf.adjoint(y, ti), x16)
+        cs = None
+        if cpu:
+            from oracle.nufft_ref import ChebyshevNufftRef, SvdNufftRef
+
+            ref = (ChebyshevNufftRef((n4, n4), 5, 1.25) if name == "chebyshev_nufft"
+                   else SvdNufftRef((n4, n4), 1.25, 6))
+            tr = ref.rescale_trajectory(sp2.clone())[None]
+            x1 = x16[:, :2].cpu()
+            cs = cpu_time(lambda: ref.adjoint(ref.forward(x1, tr), tr)) * 8
+        emit(f"{name} 2D 256^2 x 16 images, 32x4096 spiral", dict(cfg, grid_oversamp=1.25), t, cs,
+             "2 of 16 images through the oracle, scaled x8")
+        del nf
+    mps16 = synth.coil_maps(16, (n4, n4), seed=2).to(dev)
+    L = 3
+    Bf = torch.randn((L, n4, n4), dtype=torch.complex64, generator=g).to(dev)
+    Tf = torch.randn((L, sp2.shape[0]), dtype=torch.complex64, generator=g).to(dev)
+    A = sense_linop(sp2.to(dev), mps16, nufft=sigpy_nufft((n4, n4), width=4), spatial_funcs=Bf, temporal_funcs=Tf)
+    t = timed_pair(A.forward, A.adjoint, x16[0, 0])
+    emit("sense_linop 2D 256^2 x 16 coils, 3 imperfection segments, 32x4096 spiral", {"width": 4, "oversamp": 1.25},
+         t, None, "")
+    del A
+    h = torch.randn((4, 16, sp2.shape[0]), dtype=torch.complex64, generator=g).to(dev)
+    A = imperf_coil_lowrank(sp2.to(dev), mps16[:4].contiguous(), h, nufft=sigpy_nufft((n4, n4), width=4))
+    t = timed_pair(A.forward, A.adjoint, x16[0, 0])
+    emit("imperf_coil_lowrank 2D 256^2, rank 4 x 16 coils, 32x4096 spiral", {"width": 4, "oversamp": 1.25},
+         t, None, "")
 
 
 def main():
@@ -589,7 +632,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="nufft3d", choices=["nufft3d", "batched", "grad", "cg2d", "cg3d", "dcf", "rows"])
     ap.add_argument("--samples-per-item", default="auto",
-                    help="batched workload: work-item size (an int, or 'auto' for the autotuner)")
+                    help="batched workload: work-item size (an int, or 'auto' for the autotuner); default "
+                         "workload: an int overrides the plan's size heuristic")
     ap.add_argument("--headline-only", action="store_true",
                     help="default workload: skip the batched and gradient forms (profiling runs)")
     args = ap.parse_args()
@@ -617,7 +661,8 @@ def main():
     dev = torch.device("cuda:0")
     trj, mps, img = make_inputs(dev)
     t0 = time.time()
-    nf = hip_nufft((N, N, N), oversamp=OS, width=WIDTH)
+    spi = int(args.samples_per_item) if str(args.samples_per_item).isdigit() else None
+    nf = hip_nufft((N, N, N), oversamp=OS, width=WIDTH, samples_per_item=spi)
     A = sense_linop(trj, mps, nufft=nf)
     torch.cuda.synchronize()
     M = A.plan.npts

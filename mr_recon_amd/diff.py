@@ -157,20 +157,39 @@ def sense_normal(x, trj, mps, nufft: hip_nufft, dcf=None):
     return sense_adjoint(sense_forward(x, trj, mps, nufft), trj, mps, nufft, dcf)
 
 
-def _dot_re64(a, b):
-    """Re <a, b> summed in FP64 (complex128 accumulation of the complex64 products)."""
-    return torch.real(torch.sum(a.conj() * b, dtype=torch.complex128))
+class _DotRe64(Function):
+    """Re <a, b> = sum Re(conj(a) b) of complex64 vectors, accumulated in FP64 (a float64
+    scalar). Backward returns complex64 gradients (g b, g a), so the FP64 scalar does not
+    promote the vector gradients of the unrolled CG to complex128."""
+
+    @staticmethod
+    def forward(ctx, a, b):
+        ctx.save_for_backward(a, b)
+        return torch.real(torch.sum(a.conj() * b, dtype=torch.complex128))
+
+    @staticmethod
+    def backward(ctx, g):
+        a, b = ctx.saved_tensors
+        g32 = g.to(torch.float32)
+        ga = g32 * b if ctx.needs_input_grad[0] else None
+        gb = g32 * a if ctx.needs_input_grad[1] else None
+        return ga, gb
+
+
+def dot_re64(a, b):
+    """Re <a, b> of complex64 tensors as an FP64 scalar (differentiable)."""
+    return _DotRe64.apply(a, b)
 
 
 def _cg_iter(x, r, p, rz, trj, mps, nufft, dcf, lamda, inv_eig):
     """One CG step (algs.py:779-872 with P = identity) as differentiable torch ops; the
     inner products and the step sizes alpha, beta are FP64 scalars."""
     ap = sense_normal(p, trj, mps, nufft, dcf) * inv_eig + lamda * p
-    pap = _dot_re64(p, ap)
+    pap = dot_re64(p, ap)
     alpha = (rz / pap).to(torch.float32)
     x = x + alpha * p
     r = r - alpha * ap
-    rz_new = _dot_re64(r, r)
+    rz_new = dot_re64(r, r)
     p = r + (rz_new / rz).to(torch.float32) * p
     return x, r, p, rz_new
 
@@ -190,7 +209,7 @@ def unrolled_cg_sense(ksp, trj, mps, nufft: hip_nufft, num_iters: int = 10, lamd
         return x
     r = ahb - (sense_normal(x, trj, mps, nufft, dcf) * inv + lamda_l2 * x)
     p = r
-    rz = _dot_re64(r, r)
+    rz = dot_re64(r, r)
     for _ in range(num_iters):
         if checkpoint:
             x, r, p, rz = ckpt.checkpoint(_cg_iter, x, r, p, rz, trj, mps, nufft, dcf, lamda_l2, inv,
@@ -201,4 +220,4 @@ def unrolled_cg_sense(ksp, trj, mps, nufft: hip_nufft, num_iters: int = 10, lamd
 
 
 __all__ = ["SenseForwardFn", "SenseAdjointFn", "nufft_forward", "nufft_adjoint", "sense_forward",
-           "sense_adjoint", "sense_normal", "unrolled_cg_sense"]
+           "sense_adjoint", "sense_normal", "unrolled_cg_sense", "dot_re64"]

@@ -6,9 +6,20 @@ oracle:
 
 A(k) = SENSE forward (coil maps times the reference NUFFT, fourier.py:354-582);
 CG_10 = 10 CG iterations on (A^H A / max_eigen + lamda I) x = A^H y / sqrt(max_eigen)
-started at A^H y / sqrt(max_eigen) (recons.py:77-156, algs.py:779-872), lamda 1e-3,
-max_eigen 1. Shared by the fixture script (make_c5_fd.py) and the GPU test, so both
-build bit-identical inputs.
+started at A^H y / sqrt(max_eigen) (recons.py:77-156, algs.py:779-872), max_eigen = 1.01 x the
+largest eigenvalue of A(k0)^H A(k0) (2182.5, 25 FP64 power iterations on the oracle), the
+scaling CG_SENSE_recon applies by default (recons.py:119-123), and lamda 0.05: the scaled system
+then has condition number ~20. (With lamda 1e-3 its condition number is ~1000 and ten CG
+iterations amplify rounding so much that the reference's own fp32 arithmetic -- the oracle in
+fp32 -- misses the FP64 loss by 8e-3; at lamda 0.05 by 5.4e-5, recorded in c5_fd.json as
+loss_fp32_oracle.) Shared by the fixture
+script (make_c5_fd.py) and the GPU test, so both build bit-identical inputs.
+
+The trajectory gradient of the CG term is checked with ITERS_K = 3 unrolled iterations. With 10,
+fp32 CG departs from FP64 CG once the residual nears the fp32 operator's own error (relative
+residual at iteration 9: 1.75e-3 in fp32 vs 6.9e-4 in FP64), and autograd through the fp32
+oracle -- the reference's algorithm in its own precision -- gives a dL/dk that is 1.5 (relative
+L2) away from FP64 autograd; at 3 iterations the two agree to 6e-5.
 """
 
 from __future__ import annotations
@@ -20,7 +31,8 @@ import torch
 
 N, C, NINT, NSAMP = 256, 16, 64, 2048
 OS, W = 1.25, 6
-LAMDA, MAX_EIGEN, ITERS = 1e-3, 1.0, 10
+LAMDA, MAX_EIGEN, ITERS = 0.05, 2204.0, 10
+ITERS_K = 3  # unrolled iterations of the trajectory-gradient check
 H = 1e-4  # central-difference step (trajectory units: cycles per FOV; image: value units)
 
 
@@ -54,8 +66,9 @@ def build_inputs():
             "kin": (k0 + 0.05).contiguous()}
 
 
-def loss64(inp, x=None, k=None):
-    """L at (x, k) (defaults: the perturbed starting point xin, kin) in FP64."""
+def loss64(inp, x=None, k=None, iters=ITERS):
+    """L at (x, k) (defaults: the perturbed starting point xin, kin) in FP64, with ``iters``
+    CG iterations."""
     nf = nufft64()
     x = (inp["xin"] if x is None else x).to(torch.complex128)
     k = (inp["kin"] if k is None else k).to(torch.float64)
@@ -72,7 +85,7 @@ def loss64(inp, x=None, k=None):
     r = ahb - normal(xx)
     p = r
     rz = torch.real(torch.vdot(r.reshape(-1), r.reshape(-1)))
-    for _ in range(ITERS):
+    for _ in range(iters):
         ap = normal(p)
         a = rz / torch.real(torch.vdot(p.reshape(-1), ap.reshape(-1)))
         xx = xx + a * p
@@ -81,6 +94,49 @@ def loss64(inp, x=None, k=None):
         p = r + (rzn / rz) * p
         rz = rzn
     return float(dc + torch.sum(torch.abs(xx - x0) ** 2))
+
+
+def oracle32(inp, iters, grad=True):
+    """L and dL/dk at (xin, kin) by autograd through the fp32 oracle (NaN-safe kernel), the
+    inner products in FP64 as the HIP path takes them: the reference's algorithm in its own
+    precision, which sets the scale of the rounding an fp32 implementation cannot avoid.
+    Returns (loss, gradient or None)."""
+    from oracle.nufft_ref import TritonNufftRef
+
+    nf = TritonNufftRef((N, N), oversamp=OS, width=W, safe=True)
+    k = inp["kin"].clone().requires_grad_(grad)
+    x, mps, y, x0 = inp["xin"], inp["mps"], inp["y"], inp["x0"]
+
+    def fwd(v):
+        return nf.forward((mps * v)[None], k[None])[0]
+
+    def adj(w):
+        return torch.sum(nf.adjoint(w[None], k[None])[0] * mps.conj(), dim=0)
+
+    def normal(v):
+        return adj(fwd(v)) / MAX_EIGEN + LAMDA * v
+
+    def dot(a, b):
+        return torch.real(torch.sum(a.conj() * b, dtype=torch.complex128))
+
+    ahb = adj(y) / math.sqrt(MAX_EIGEN)
+    xx = ahb
+    r = ahb - normal(xx)
+    p = r
+    rz = dot(r, r)
+    for _ in range(iters):
+        ap = normal(p)
+        a = (rz / dot(p, ap)).to(torch.float32)
+        xx = xx + a * p
+        r = r - a * ap
+        rzn = dot(r, r)
+        p = r + (rzn / rz).to(torch.float32) * p
+        rz = rzn
+    loss = torch.sum(torch.abs(fwd(x) - y) ** 2) + torch.sum(torch.abs(xx - x0) ** 2)
+    if not grad:
+        return float(loss), None
+    loss.backward()
+    return float(loss.detach()), k.grad.to(torch.float64)
 
 
 def smooth_direction(k, v):

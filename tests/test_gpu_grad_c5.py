@@ -1,0 +1,186 @@
+"""GPU parity of the trajectory-gradient workload at full size (2D 256^2 image, 16 coils, 64
+spiral interleaves x 2048 samples, w 6, s 1.25):
+
+    L(x, k) = ||A(k) x - y||^2 + ||CG_n(A(k), y) - x0||^2
+
+defined in tests/golden/c5_problem.py. Checked against
+  * FP64 central differences (step 1e-4) of L evaluated with the CPU oracle, committed in
+    tests/golden/c5_fd.json by tests/golden/make_c5_fd.py: the loss and one image direction
+    with the workload's n = 10 CG iterations; five trajectory directions (three dense, two
+    confined to one interleave) with n = 3 (ITERS_K; with 10, fp32 CG departs from FP64 CG
+    near convergence and autograd through the fp32 oracle itself misses the FP64 dL/dk by
+    1.5 relative L2, so no fp32 implementation can be held to FP64 there);
+  * the FP64 oracle's own gradients of the data term: dL/dx = 2 A^H (A x - y) and dL/dk by
+    autograd through the oracle's pure-torch NUFFT (NaN-safe masking).
+Both the fused path (SENSE autograd functions, checkpointed unrolled CG) and the plain path
+(per-coil differentiable NUFFTs composed in Python, no checkpointing) are checked.
+"""
+
+import json
+import os
+import sys
+
+import pytest
+import torch
+
+from helpers import rel_l2
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import c5_problem as P  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+# Tolerances (relative). FP64 central differences have truncation error ~h^2 (third derivative)
+# (below 1e-7 here); the fp32 GPU path carries the rounding of complex64 NUFFTs through the CG
+# iterations, so the loss is bounded by max(TOL_LOSS, 5 x the fp32 oracle's own deviation).
+# Trajectory directions: the error is taken relative to max(|fd|, ||g * v||_2), the size of a
+# projection whose terms do not cancel, and bounded by max(TOL_DIR_K, 5 x the fp32 oracle's own
+# error on the same projection).
+TOL_LOSS = 2e-5
+TOL_DIR_K = 1e-4
+TOL_DIR_X = 1e-5
+TOL_GX = 1e-5
+TOL_GK_DATA = 1e-4
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def c5(dev):
+    with open(os.path.join(HERE, "golden", "c5_fd.json")) as f:
+        fix = json.load(f)
+    inp = P.build_inputs()
+    ys = float(inp["y"].abs().to(torch.float64).sum())
+    assert abs(ys - fix["y_abs_sum"]) < 1e-6 * fix["y_abs_sum"], "inputs differ from the fixture's"
+    return inp, fix
+
+
+def _fast(inp, dev, iters, data_only=False):
+    from mr_recon_amd import diff
+    from mr_recon_amd.fourier import hip_nufft
+
+    nf = hip_nufft((P.N, P.N), oversamp=P.OS, width=P.W)
+    mps, y, x0 = inp["mps"].to(dev), inp["y"].to(dev), inp["x0"].to(dev)
+    x = inp["xin"].to(dev).requires_grad_(True)
+    k = inp["kin"].to(dev).requires_grad_(True)
+    loss = torch.sum(torch.abs(diff.sense_forward(x, k, mps, nf) - y) ** 2)
+    if not data_only:
+        rec = diff.unrolled_cg_sense(y, k, mps, nf, num_iters=iters, lamda_l2=P.LAMDA, max_eigen=P.MAX_EIGEN,
+                                     checkpoint=True)
+        loss = loss + torch.sum(torch.abs(rec - x0) ** 2)
+    loss.backward()
+    return float(loss), x.grad.cpu(), k.grad.cpu()
+
+
+def _plain(inp, dev, iters):
+    from mr_recon_amd import diff
+    from mr_recon_amd.fourier import hip_nufft
+
+    nf = hip_nufft((P.N, P.N), oversamp=P.OS, width=P.W)
+    mps, y, x0 = inp["mps"].to(dev), inp["y"].to(dev), inp["x0"].to(dev)
+    x = inp["xin"].to(dev).requires_grad_(True)
+    k = inp["kin"].to(dev).requires_grad_(True)
+    C = P.C
+
+    def A(v):
+        return [diff.nufft_forward(mps[c] * v, k, nf) for c in range(C)]
+
+    def AH(ys):
+        return sum(mps[c].conj() * diff.nufft_adjoint(ys[c], k, nf) for c in range(C))
+
+    dot = diff.dot_re64
+
+    dc = sum(torch.sum(torch.abs(a - y[c]) ** 2) for c, a in enumerate(A(x)))
+    inv = 1.0 / P.MAX_EIGEN
+    xx = AH([y[c] for c in range(C)]) * P.MAX_EIGEN ** -0.5
+    r = xx - (AH(A(xx)) * inv + P.LAMDA * xx)
+    p = r
+    rz = dot(r, r)
+    for _ in range(iters):
+        ap = AH(A(p)) * inv + P.LAMDA * p
+        a = (rz / dot(p, ap)).to(torch.float32)
+        xx = xx + a * p
+        r = r - a * ap
+        rzn = dot(r, r)
+        p = r + (rzn / rz).to(torch.float32) * p
+        rz = rzn
+    loss = dc + torch.sum(torch.abs(xx - x0) ** 2)
+    loss.backward()
+    return float(loss), x.grad.cpu(), k.grad.cpu()
+
+
+def _loss_error(tag, loss, ref, ref32, iters):
+    """Relative error of the loss, bounded by max(TOL_LOSS, 5 x the fp32 oracle's own)."""
+    e, e32 = abs(loss - ref) / abs(ref), abs(ref32 - ref) / abs(ref)
+    _say(f"C5 {tag} ({iters} CG iterations): loss {loss:.8e} vs FP64 {ref:.8e} (rel {e:.2e}; fp32 oracle {e32:.2e})")
+    assert e < max(TOL_LOSS, 5 * e32)
+
+
+def _check_loss_and_image(tag, loss, gx, fix):
+    _loss_error(tag, loss, fix["loss"], fix["loss_fp32_oracle"], P.ITERS)
+    vx = P.x_direction()
+    an = float(torch.sum(torch.real(gx.to(torch.complex128).conj() * vx)))
+    fd = fix["x"]["random"]["fd"]
+    ex = abs(an - fd) / abs(fd)
+    _say(f"C5 {tag}: dL/dx . v = {an:.8e}, FP64 central difference {fd:.8e} (rel {ex:.2e})")
+    assert ex < TOL_DIR_X
+
+
+def _check_trajectory(tag, loss, gk, inp, fix):
+    _loss_error(tag, loss, fix["loss_k"], fix["loss_k_fp32_oracle"], fix["k_iters"])
+    for name, v in P.k_directions(inp["kin"]):
+        ent = fix["k"][name]
+        fd = ent["fd"]
+        prod = gk.to(torch.float64) * v
+        an = float(torch.sum(prod))
+        scale = max(abs(fd), float(torch.linalg.norm(prod)))
+        e = abs(an - fd) / scale
+        e32 = abs(ent["fp32_oracle"] - fd) / scale
+        bound = max(TOL_DIR_K, 5 * e32)
+        _say(f"C5 {tag}: dL/dk . v[{name}] = {an:.8e}, FP64 central difference {fd:.8e} (err {e:.2e}; "
+             f"fp32 oracle {e32:.2e}; bound {bound:.2e})")
+        assert e < bound, name
+
+
+def test_c5_fused_path_vs_fp64_finite_differences(dev, c5):
+    inp, fix = c5
+    _say("C5: fused path (SENSE autograd + checkpointed unrolled CG), 256^2 x 16 coils, 64 x 2048")
+    loss, gx, _ = _fast(inp, dev, P.ITERS)
+    _check_loss_and_image("fused", loss, gx, fix)
+    loss, _, gk = _fast(inp, dev, fix["k_iters"])
+    _check_trajectory("fused", loss, gk, inp, fix)
+
+
+def test_c5_plain_path_vs_fp64_finite_differences(dev, c5):
+    inp, fix = c5
+    _say("C5: plain path (per-coil differentiable NUFFTs, Python CG), 256^2 x 16 coils, 64 x 2048")
+    loss, gx, _ = _plain(inp, dev, P.ITERS)
+    _check_loss_and_image("plain", loss, gx, fix)
+    loss, _, gk = _plain(inp, dev, fix["k_iters"])
+    _check_trajectory("plain", loss, gk, inp, fix)
+
+
+def test_c5_data_term_gradients_vs_fp64_oracle(dev, c5):
+    """Full gradient vectors of the data term: dL/dx = 2 A^H (A x - y) and dL/dk by autograd
+    through the FP64 oracle."""
+    inp, _ = c5
+    _say("C5: data-term gradients, fused path vs FP64 oracle (full vectors)")
+    loss, gx, gk = _fast(inp, dev, 0, data_only=True)
+    from oracle.nufft_ref import TritonNufftRef
+
+    nf = TritonNufftRef((P.N, P.N), oversamp=P.OS, width=P.W, dtype=torch.float64, coords64=True, safe=True)
+    mps = inp["mps"].to(torch.complex128)
+    y = inp["y"].to(torch.complex128)
+    x = inp["xin"].to(torch.complex128)
+    k = inp["kin"].to(torch.float64).requires_grad_(True)
+    res = nf.forward((mps * x)[None], k[None])[0] - y
+    l64 = torch.sum(torch.abs(res) ** 2)
+    l64.backward()
+    with torch.no_grad():
+        gx64 = 2 * torch.sum(nf.adjoint(res.detach()[None], k.detach()[None])[0] * mps.conj(), dim=0)
+    e_l, e_x, e_k = abs(loss - float(l64)) / float(l64), rel_l2(gx, gx64), rel_l2(gk.to(torch.float64), k.grad)
+    _say(f"C5 data term: loss rel {e_l:.2e}, dL/dx rel L2 {e_x:.2e}, dL/dk rel L2 {e_k:.2e}")
+    assert e_l < 1e-5 and e_x < TOL_GX and e_k < TOL_GK_DATA
