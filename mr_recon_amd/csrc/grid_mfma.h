@@ -48,8 +48,10 @@ namespace mra {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-template <int P, int NWV = MRA_MFMA_NW>
+template <int P, int NWV = MRA_MFMA_NW, int E1V = MRA_MFMA_E1, int PEV = P>
 struct MfmaGrid {
+  static constexpr int NP = P;                 // patch points per dim of the plan's weights
+  static constexpr int PE = PEV;               // patch points per dim the samples reach
   static constexpr int T = 8;                  // 3D tile edge
   static constexpr int SUB = T + P - 1;        // subgrid edge
   static constexpr int PP = SUB * SUB;         // dense points per plane
@@ -59,13 +61,13 @@ struct MfmaGrid {
   static constexpr int NC = NW * NCW;          // coils per block
   static constexpr int PV = 8 / NCW;           // values of p2 per 16 rows
   static constexpr int PF = 2 * NCW;           // floats per grid point in the ring
-  static constexpr int E1 = MRA_MFMA_E1;       // cell extent along dim 1 (corner rows)
+  static constexpr int E1 = E1V;               // cell extent along dim 1 (corner rows)
   static constexpr int E2 = MRA_MFMA_E2;       // cell extent along dim 2 (corner positions)
-  static constexpr int E1P = E1 + P - 1;       // region extent along dim 1
-  static constexpr int E2P = E2 + P - 1;       // region extent along dim 2
+  static constexpr int E1P = E1 + PE - 1;      // region extent along dim 1
+  static constexpr int E2P = E2 + PE - 1;      // region extent along dim 2
   static constexpr int RB = (E2P + PV - 1) / PV;  // row blocks
   static constexpr int CWS = RB;               // floats per (sample, p2 mod PV) of dim-2 weights
-  static constexpr int NJ = P * E1P;           // (p0, p1) columns
+  static constexpr int NJ = PE * E1P;          // (p0, p1) columns
   static constexpr int CB = (NJ + 15) / 16;    // column blocks
   static constexpr int CPW = CB * 16 / NW;     // B columns staged per wave
   static constexpr int CH = 64;                // samples per stage
@@ -79,15 +81,17 @@ struct MfmaGrid {
   static_assert(NCW == 2 || NCW == 4, "2 or 4 coils per wave");
   static_assert(NW == 1 || NW == 2 || NW == 4, "1, 2 or 4 waves per block");
   static_assert(CPW % 4 == 0, "B columns are staged four at a time");
+  static_assert(PE >= 1 && PE <= P, "patch points reached");
   // cells must be contiguous in the plan's order (a0, a1 / 2, a2 / 4, a1 % 2, a2 % 4)
-  static_assert((E1 == 2 && (E2 == 4 || E2 == 8)) || (E2 == 8 && (E1 == 4 || E1 == 8)), "cell shape");
+  static_assert((E1 == 1 && E2 == 4) || (E1 == 2 && (E2 == 4 || E2 == 8)) || (E2 == 8 && (E1 == 4 || E1 == 8)),
+                "cell shape");
 };
 
 // Planes lo .. hi-1 of this wave's ring are final: store them to scratch (dense) and
 // clear their slots. Planes the ring never held (p >= lo + P) are stored as zeros.
-template <int P>
+template <class M>
 __device__ __forceinline__ void mfma_retire(int lo, int hi, int nc, float* ring, float2* __restrict__ dst) {
-  using M = MfmaGrid<P>;
+  constexpr int P = M::NP;
   constexpr int NE = M::NCW * M::PP;
   constexpr int NIT = (NE + 63) / 64;
   const int lane = threadIdx.x & 63;
@@ -121,9 +125,9 @@ __device__ __forceinline__ void mfma_retire(int lo, int hi, int nc, float* ring,
 // B operand columns W*CPW .. (W+1)*CPW-1 of this lane's sample: column j = (p0, p1) =
 // (j / E1P, j % E1P) holds w0[p0] * w1[p1 - off1] (0 outside the patch or past NJ).
 // Layout [cb][sample][16]; four columns per 128-bit store.
-template <int P, int W, int NWV>
-__device__ __forceinline__ void mfma_stage_b(const float (&rw)[3 * P], int off1, float* sbw, int lane) {
-  using M = MfmaGrid<P, NWV>;
+template <class M, int W>
+__device__ __forceinline__ void mfma_stage_b(const float (&rw)[3 * M::NP], int off1, float* sbw, int lane) {
+  constexpr int P = M::NP;
 #pragma unroll
   for (int c4 = 0; c4 < M::CPW / 4; ++c4) {
     float v[4];
@@ -134,7 +138,7 @@ __device__ __forceinline__ void mfma_stage_b(const float (&rw)[3 * P], int off1,
       float w1 = 0.0f;
 #pragma unroll
       for (int o = 0; o < M::E1; ++o)
-        if (p1 - o >= 0 && p1 - o < P) w1 = off1 == o ? rw[P + p1 - o] : w1;
+        if (p1 - o >= 0 && p1 - o < M::PE) w1 = off1 == o ? rw[P + p1 - o] : w1;
       v[u] = j < M::NJ ? rw[p0] * w1 : 0.0f;
     }
     const int j = W * M::CPW + 4 * c4;
@@ -146,9 +150,9 @@ __device__ __forceinline__ void mfma_stage_b(const float (&rw)[3 * P], int off1,
 // (rb, cb) are p2 = c2 + PV rb + 4q / PF, floats (4q mod PF) .. +3 of that point; column
 // (p0, p1) = (j / E1P, j % E1P), j = 16 cb + lane % 16. Columns past NJ and rows past E2P
 // carry zeros and go to the lane's dummy slot.
-template <int P>
+template <class M>
 struct CellFlush {
-  using M = MfmaGrid<P>;
+  static constexpr int P = M::NP;
   v4f* ptr[M::RB][M::CB];
   v4f old[M::RB][M::CB];
 
@@ -185,25 +189,16 @@ struct CellFlush {
   }
 };
 
-template <int P, int NWV>
-__global__ __launch_bounds__(64 * NWV) void grid_mfma_kernel(
-    const float2* __restrict__ ksp, const float* __restrict__ dcf, const float* __restrict__ wsort,
-    const int* __restrict__ lbase, const int* __restrict__ perm, const Geom* __restrict__ geoms,
-    const Item* __restrict__ items, int nitems, int ngroups, float2* __restrict__ scratch) {
-  using M = MfmaGrid<P, NWV>;
+// One (item, coil group) with configuration M: the work of grid_mfma_kernel below. The LDS
+// buffers come from the kernel, sized for every configuration it may pick.
+template <class M>
+__device__ __forceinline__ void grid_mfma_item(const float2* __restrict__ ksp, const float* __restrict__ dcf,
+                                               const float* __restrict__ wsort, const int* __restrict__ lbase,
+                                               const int* __restrict__ perm, const Geom& g, const Item& it,
+                                               int iidx, int grp, float2* __restrict__ scratch, float* ring_all,
+                                               float* sbw, float* scw, float* sy) {
+  constexpr int P = M::NP;
   constexpr int SUB = M::SUB, CH = M::CH, RB = M::RB, CB = M::CB, PF = M::PF;
-  __shared__ __attribute__((aligned(16))) float ring_all[M::NW * M::RING];
-  __shared__ __attribute__((aligned(16))) float sbw[CB * CH * 16];         // B operands [cb][s][16]
-  __shared__ __attribute__((aligned(16))) float scw[CH * M::PV * M::CWS];  // dim-2 weights [s][p2 mod PV][rb]
-  __shared__ __attribute__((aligned(16))) float sy[M::NW * CH * PF];       // k-space [wave][s][coil][re,im]
-
-  int iidx, grp;
-  item_of_block(blockIdx.x, ngroups, iidx, grp);
-  if (iidx >= nitems) return;
-  const Item it = items[iidx];
-  if (it.w <= 0 || it.y < 0) return;  // non-finite samples contribute nothing
-  const Geom g = geoms[it.x];
-  if (grp * M::NC >= g.nbatch) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int cw0 = grp * M::NC + wave * M::NCW;  // this wave's first coil
   const int cnt = it.w;
@@ -272,10 +267,10 @@ __global__ __launch_bounds__(64 * NWV) void grid_mfma_kernel(
       key = (l0 << 16) | ((l1 / M::E1) << 8) | (l2 / M::E2);
       const int off1 = l1 % M::E1, off2 = l2 % M::E2;
       switch (wave) {
-        case 0: mfma_stage_b<P, 0, NWV>(rw, off1, sbw, lane); break;
-        case 1: mfma_stage_b<P, 1 % M::NW, NWV>(rw, off1, sbw, lane); break;
-        case 2: mfma_stage_b<P, 2 % M::NW, NWV>(rw, off1, sbw, lane); break;
-        default: mfma_stage_b<P, 3 % M::NW, NWV>(rw, off1, sbw, lane); break;
+        case 0: mfma_stage_b<M, 0>(rw, off1, sbw, lane); break;
+        case 1: mfma_stage_b<M, 1 % M::NW>(rw, off1, sbw, lane); break;
+        case 2: mfma_stage_b<M, 2 % M::NW>(rw, off1, sbw, lane); break;
+        default: mfma_stage_b<M, 3 % M::NW>(rw, off1, sbw, lane); break;
       }
       if (wave == M::NW - 1) {
         // dim-2 weights on the region: cw[p2] = w2[p2 - off2], stored by p2 mod PV
@@ -286,7 +281,7 @@ __global__ __launch_bounds__(64 * NWV) void grid_mfma_kernel(
             const int t = M::PV * rb + ps - off2;
             float v = 0.0f;
 #pragma unroll
-            for (int u = 0; u < P; ++u) v = t == u ? rw[2 * P + u] : v;
+            for (int u = 0; u < M::PE; ++u) v = t == u ? rw[2 * P + u] : v;
             scw[(lane * M::PV + ps) * M::CWS + rb] = v;
           }
       }
@@ -343,7 +338,7 @@ __global__ __launch_bounds__(64 * NWV) void grid_mfma_kernel(
         // a new cell; the previous one was written back unless it ran to a stage end
         const int nk = __builtin_amdgcn_readlane(key, s);
         if (pending) {
-          CellFlush<P> f;
+          CellFlush<M> f;
           f.locate(ring, ckey, j0, loff, prow, lane);
           f.read();
           f.write(acc);
@@ -354,7 +349,7 @@ __global__ __launch_bounds__(64 * NWV) void grid_mfma_kernel(
           for (int cb = 0; cb < CB; ++cb) acc[rb][cb] = v4f{0.f, 0.f, 0.f, 0.f};
         const int pl = nk >> 16;
         if (pl > lo) {
-          mfma_retire<P>(lo, pl, nc, ring, dst);
+          mfma_retire<M>(lo, pl, nc, ring, dst);
           lo = pl;
         }
         ckey = nk;
@@ -379,7 +374,7 @@ __global__ __launch_bounds__(64 * NWV) void grid_mfma_kernel(
         if (s2 < n) load_ops(nxt, s2, s3 - s2);
       };
       if (last) {
-        CellFlush<P> f;
+        CellFlush<M> f;
         f.locate(ring, ckey, j0, loff, prow, lane);
         f.read();
         products();
@@ -401,12 +396,63 @@ __global__ __launch_bounds__(64 * NWV) void grid_mfma_kernel(
     __syncthreads();  // the stage is rewritten next
   }
   if (pending) {
-    CellFlush<P> f;
+    CellFlush<M> f;
     f.locate(ring, ckey, j0, loff, prow, lane);
     f.read();
     f.write(acc);
   }
-  mfma_retire<P>(lo, SUB, nc, ring, dst);
+  mfma_retire<M>(lo, SUB, nc, ring, dst);
+}
+
+
+// Kernel: one block per (item, coil group). Items whose samples never reach the last patch
+// point of a dimension (their weight there is exactly 0: t - w/2 off the grid integers, every
+// sample but the k-space centre's for integer widths) run with P - 1 points per dimension and
+// cells of 1 x 4 corners: at P = 5 the (p0, p1) columns are 4 x 4 = 16, one matrix block
+// instead of two, so a sample costs half the matrix instructions and B operands. Items with
+// such a sample keep P points and 2 x 4 cells. The decision is per item, from one pass over
+// the last weights of its samples (L2-resident: the item's coil groups run back to back on
+// one XCD). Both configurations write the same scratch subgrids.
+#ifndef MRA_MFMA_SHORT
+#define MRA_MFMA_SHORT 1
+#endif
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+template <int P, int NWV>
+__global__ __launch_bounds__(64 * NWV) void grid_mfma_kernel(
+    const float2* __restrict__ ksp, const float* __restrict__ dcf, const float* __restrict__ wsort,
+    const int* __restrict__ lbase, const int* __restrict__ perm, const Geom* __restrict__ geoms,
+    const Item* __restrict__ items, int nitems, int ngroups, float2* __restrict__ scratch) {
+  using MF = MfmaGrid<P, NWV>;
+  constexpr bool kShort = MRA_MFMA_SHORT && P >= 4;
+  using MS = MfmaGrid<P, NWV, kShort ? 1 : MRA_MFMA_E1, kShort ? P - 1 : P>;
+  static_assert(MF::NC == MS::NC, "same coils per block");
+  __shared__ __attribute__((aligned(16))) float ring_all[cmax(MF::NW * MF::RING, MS::NW * MS::RING)];
+  __shared__ __attribute__((aligned(16))) float sbw[cmax(MF::CB, MS::CB) * MF::CH * 16];  // B operands
+  __shared__ __attribute__((aligned(16))) float scw[MF::CH * MF::PV * cmax(MF::CWS, MS::CWS)];  // dim-2 weights
+  __shared__ __attribute__((aligned(16))) float sy[MF::NW * MF::CH * MF::PF];             // k-space
+
+  int iidx, grp;
+  item_of_block(blockIdx.x, ngroups, iidx, grp);
+  if (iidx >= nitems) return;
+  const Item it = items[iidx];
+  if (it.w <= 0 || it.y < 0) return;  // non-finite samples contribute nothing
+  const Geom g = geoms[it.x];
+  if (grp * MF::NC >= g.nbatch) return;
+  bool full = true;
+  if constexpr (kShort) {
+    const float* w0 = wsort + (g.samp_off + it.z) * (3 * P);
+    int edge = 0;
+    for (int s = threadIdx.x; s < it.w; s += 64 * NWV) {
+      const float* w = w0 + (long long)s * (3 * P);
+      edge |= (w[P - 1] != 0.0f) | (w[2 * P - 1] != 0.0f) | (w[3 * P - 1] != 0.0f);
+    }
+    full = __syncthreads_or(edge) != 0;
+  }
+  if (full)
+    grid_mfma_item<MF>(ksp, dcf, wsort, lbase, perm, g, it, iidx, grp, scratch, ring_all, sbw, scw, sy);
+  else
+    grid_mfma_item<MS>(ksp, dcf, wsort, lbase, perm, g, it, iidx, grp, scratch, ring_all, sbw, scw, sy);
 }
 
 }  // namespace mra

@@ -1079,6 +1079,56 @@ __global__ __launch_bounds__(kThreads) void grid_reduce_kernel(float2* __restric
 // One block per (item, 256*K-sample sub-slice); all coils in the block, the
 // coil subgrid double-buffered in LDS; weights and derivative weights are
 // computed on the fly from the sorted coordinates.
+// Value gather and its per-dimension derivative gathers over the first PE points of every
+// dimension: sd[d] = sum_p (prod_{e != d} w_e[p_e]) dw_d[p_d] v[p] (see interp_grad_kernel).
+// With PE = P - 1 the skipped terms carry weight and derivative 0 (the last point lies
+// outside the support), so the sums are unchanged.
+template <int D, int P, int SUB, int PE>
+__device__ __forceinline__ void grad_sums(const float2* sb, const float (&w)[D][P], const float (&dw)[D][P],
+                                          float2 (&sd)[D]) {
+  if constexpr (D == 3) {
+#pragma unroll
+    for (int i = 0; i < PE; ++i) {
+      float2 cA = {0.f, 0.f}, cB = {0.f, 0.f}, cC = {0.f, 0.f};
+#pragma unroll
+      for (int j = 0; j < PE; ++j) {
+        float2 rA = {0.f, 0.f}, rB = {0.f, 0.f};
+#pragma unroll
+        for (int l = 0; l < PE; ++l) {
+          const float2 v = sb[(i * SUB + j) * SUB + l];
+          rA.x += w[2][l] * v.x;  rA.y += w[2][l] * v.y;
+          rB.x += dw[2][l] * v.x; rB.y += dw[2][l] * v.y;
+        }
+        cA.x += w[1][j] * rA.x;  cA.y += w[1][j] * rA.y;
+        cB.x += dw[1][j] * rA.x; cB.y += dw[1][j] * rA.y;
+        cC.x += w[1][j] * rB.x;  cC.y += w[1][j] * rB.y;
+      }
+      sd[0].x += dw[0][i] * cA.x; sd[0].y += dw[0][i] * cA.y;
+      sd[1].x += w[0][i] * cB.x;  sd[1].y += w[0][i] * cB.y;
+      sd[2].x += w[0][i] * cC.x;  sd[2].y += w[0][i] * cC.y;
+    }
+  } else if constexpr (D == 2) {
+#pragma unroll
+    for (int i = 0; i < PE; ++i) {
+      float2 rA = {0.f, 0.f}, rB = {0.f, 0.f};
+#pragma unroll
+      for (int l = 0; l < PE; ++l) {
+        const float2 v = sb[i * SUB + l];
+        rA.x += w[1][l] * v.x;  rA.y += w[1][l] * v.y;
+        rB.x += dw[1][l] * v.x; rB.y += dw[1][l] * v.y;
+      }
+      sd[0].x += dw[0][i] * rA.x; sd[0].y += dw[0][i] * rA.y;
+      sd[1].x += w[0][i] * rB.x;  sd[1].y += w[0][i] * rB.y;
+    }
+  } else {
+#pragma unroll
+    for (int l = 0; l < PE; ++l) {
+      const float2 v = sb[l];
+      sd[0].x += dw[0][l] * v.x; sd[0].y += dw[0][l] * v.y;
+    }
+  }
+}
+
 template <int D, int P, int T, int K>
 __global__ __launch_bounds__(kThreads) void interp_grad_kernel(const float2* __restrict__ grid,
                                                                const float2* __restrict__ a,
@@ -1146,10 +1196,15 @@ __global__ __launch_bounds__(kThreads) void interp_grad_kernel(const float2* __r
   }
 
   float acc[K][D];
+  bool edge = false;  // some sample reaches the last patch point of a dimension
 #pragma unroll
   for (int k = 0; k < K; ++k)
 #pragma unroll
-    for (int d = 0; d < D; ++d) acc[k][d] = 0.f;
+    for (int d = 0; d < D; ++d) {
+      acc[k][d] = 0.f;
+      edge = edge || w[k][d][P - 1] != 0.0f;
+    }
+  const bool full = __any(edge);  // wave-uniform
 
   const float2* gsrc = grid + g.grid_off;
   const float2* asrc = a + g.ksp_off;
@@ -1176,46 +1231,13 @@ __global__ __launch_bounds__(kThreads) void interp_grad_kernel(const float2* __r
       float2 sd[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) sd[d] = make_float2(0.f, 0.f);
-      if constexpr (D == 3) {
-#pragma unroll
-        for (int i = 0; i < P; ++i) {
-          float2 cA = {0.f, 0.f}, cB = {0.f, 0.f}, cC = {0.f, 0.f};
-#pragma unroll
-          for (int j = 0; j < P; ++j) {
-            float2 rA = {0.f, 0.f}, rB = {0.f, 0.f};
-#pragma unroll
-            for (int l = 0; l < P; ++l) {
-              const float2 v = sb[(i * SUB + j) * SUB + l];
-              rA.x += w[k][2][l] * v.x;  rA.y += w[k][2][l] * v.y;
-              rB.x += dw[k][2][l] * v.x; rB.y += dw[k][2][l] * v.y;
-            }
-            cA.x += w[k][1][j] * rA.x;  cA.y += w[k][1][j] * rA.y;
-            cB.x += dw[k][1][j] * rA.x; cB.y += dw[k][1][j] * rA.y;
-            cC.x += w[k][1][j] * rB.x;  cC.y += w[k][1][j] * rB.y;
-          }
-          sd[0].x += dw[k][0][i] * cA.x; sd[0].y += dw[k][0][i] * cA.y;
-          sd[1].x += w[k][0][i] * cB.x;  sd[1].y += w[k][0][i] * cB.y;
-          sd[2].x += w[k][0][i] * cC.x;  sd[2].y += w[k][0][i] * cC.y;
-        }
-      } else if constexpr (D == 2) {
-#pragma unroll
-        for (int i = 0; i < P; ++i) {
-          float2 rA = {0.f, 0.f}, rB = {0.f, 0.f};
-#pragma unroll
-          for (int l = 0; l < P; ++l) {
-            const float2 v = sb[i * SUB + l];
-            rA.x += w[k][1][l] * v.x;  rA.y += w[k][1][l] * v.y;
-            rB.x += dw[k][1][l] * v.x; rB.y += dw[k][1][l] * v.y;
-          }
-          sd[0].x += dw[k][0][i] * rA.x; sd[0].y += dw[k][0][i] * rA.y;
-          sd[1].x += w[k][0][i] * rB.x;  sd[1].y += w[k][0][i] * rB.y;
-        }
+      if constexpr (D == 3 && P >= 9) {  // two unrolled copies would not fit the registers
+        grad_sums<D, P, SUB, P>(sb, w[k], dw[k], sd);
       } else {
-#pragma unroll
-        for (int l = 0; l < P; ++l) {
-          const float2 v = sb[l];
-          sd[0].x += dw[k][0][l] * v.x; sd[0].y += dw[k][0][l] * v.y;
-        }
+        if (full)
+          grad_sums<D, P, SUB, P>(sb, w[k], dw[k], sd);
+        else
+          grad_sums<D, P, SUB, P - 1>(sb, w[k], dw[k], sd);
       }
 #pragma unroll
       for (int d = 0; d < D; ++d) acc[k][d] += av.x * sd[d].x + av.y * sd[d].y;

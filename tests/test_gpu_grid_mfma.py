@@ -71,6 +71,29 @@ def test_mfma_grid_vs_oracle_and_lane_kernel(dev, width, C):
     assert rel_l2(mfma, lane) < TOL
 
 
+@pytest.mark.parametrize("width", [4.0, 5.0, 6.0, 3.5, 7.0])
+@pytest.mark.parametrize("frac", [0.0, 0.002, 1.0])
+def test_mfma_grid_short_and_full_items(dev, width, frac):
+    """An item whose samples never reach the last patch point of a dimension (t - w/2 off the
+    grid integers) is gridded with P - 1 points per dimension and 1 x 4 corner cells; one
+    with such a sample keeps P points and 2 x 4 cells. No sample, a few (scattered over
+    items) and every sample on the lattice, against the oracle and the lane kernel."""
+    G = 24
+    rng = np.random.default_rng(int(width * 10) + int(frac * 1000))
+    n = 6000
+    k = rng.random((n, 3)) * G
+    on = rng.random(n) < frac
+    k[on] = np.floor(k[on]) + (width / 2 - np.floor(width / 2))
+    k = torch.from_numpy(k.astype(np.float32))
+    g = torch.Generator().manual_seed(3)
+    y = torch.randn((6, n), dtype=torch.complex64, generator=g)
+    beta = 2.3 * width
+    mfma, lane = _grid_both(dev, k, y, None, G, width, beta, S=500)
+    ref = interp_ref.grid(y, k, (G,) * 3, width, beta=beta)
+    assert rel_l2(mfma, ref) < TOL
+    assert rel_l2(mfma, lane) < TOL
+
+
 @pytest.mark.parametrize("S", [64, 100, 1000])
 def test_mfma_grid_item_boundaries(dev, S):
     """Items of S samples cut cells at arbitrary places; each item flushes its own part."""

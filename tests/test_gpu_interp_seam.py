@@ -95,6 +95,32 @@ def test_interpolate_adjoint_values_and_gradients(dev, case):
     assert rel_l2(ld.grad.to(torch.float64), l64.grad) < 1e-3
 
 
+@pytest.mark.parametrize("G,w", [((40, 36), 6.0), ((16, 18, 20), 4.0)])
+def test_locs_gradient_on_the_lattice(dev, G, w):
+    """Samples with t - w/2 on grid integers reach floor(w) + 1 points per dimension (the
+    support edge |u| = 1 included, where the weight is I0(0) and its derivative finite);
+    the rest reach one fewer. A third of the samples on the lattice: values and the locs
+    gradient against FP64 oracle autograd."""
+    from mr_recon_amd.hip_interp import interpolate
+
+    locs, vals, samp = _inputs(G, True, seed=77, npts=900)
+    g = torch.Generator().manual_seed(78)
+    on = torch.rand(locs.shape[0], generator=g) < 0.33
+    lattice = torch.floor(locs) + (w / 2 - math.floor(w / 2))
+    locs = torch.where(on[:, None], lattice, locs)
+    kp = {"beta": _beta(w)}
+    vd = vals.to(dev)
+    ld = locs.to(dev).requires_grad_(True)
+    out = interpolate(vd, ld, w, kernel_params=kp)
+    assert rel_l2(out, interp_ref.ungrid(vals, locs, w, beta=kp["beta"])) < 1e-5
+    _loss(out, samp.to(dev)).backward()
+    l64 = locs.to(torch.float64).requires_grad_(True)
+    o64 = interp_ref.ungrid(vals.to(torch.complex128), l64, w, beta=kp["beta"], dtype=torch.float64,
+                            coords64=True, safe=True)
+    _loss(o64, samp.to(o64.dtype)).backward()
+    assert rel_l2(ld.grad.to(torch.float64), l64.grad) < 1e-3
+
+
 def test_interpolate_pair_is_adjoint(dev):
     """<interpolate(v), s> = <v, interpolate_adjoint(s)> through the seam (2D complex)."""
     from mr_recon_amd.hip_interp import interpolate, interpolate_adjoint
