@@ -241,10 +241,12 @@ def fused_supported(grid_size) -> bool:
 
 def pad_fft(x: torch.Tensor, im_size, grid_size, smaps: torch.Tensor | None = None,
             apod: torch.Tensor | None = None, scale: float = 1.0, center=None,
-            img_index: torch.Tensor | None = None, nq: int | None = None) -> torch.Tensor:
+            img_index: torch.Tensor | None = None, nq: int | None = None,
+            out: torch.Tensor | None = None) -> torch.Tensor:
     """Spectrum of the zero-padded images: FFT(pad_expand(x, smaps, apod, scale)) as a
-    new grid tensor (nx[, C], *G), unnormalised, unshifted layout. With ``img_index``
-    (batched problems), grid q pads image img_index[q] times map q (nq grids)."""
+    grid tensor (nx[, C], *G), unnormalised, unshifted layout: new, or written into
+    ``out`` (contiguous complex64 with as many elements). With ``img_index`` (batched
+    problems), grid q pads image img_index[q] times map q (nq grids)."""
     from .gridfft import _fftn, _lds_fft
 
     D = len(im_size)
@@ -253,7 +255,13 @@ def pad_fft(x: torch.Tensor, im_size, grid_size, smaps: torch.Tensor | None = No
         if img_index is not None:
             raise ValueError("pad_fft: batched padding needs a grid with an LDS FFT plan")
         g = pad_expand(x, im_size, G, smaps=smaps, apod=apod, scale=scale, center=center)
-        return _fftn(g.reshape(-1, *G), D).reshape(g.shape)
+        g = _fftn(g.reshape(-1, *G), D).reshape(g.shape)
+        if out is None:
+            return g
+        if out.numel() != g.numel() or out.dtype != torch.complex64 or not out.is_contiguous():
+            raise ValueError("pad_fft: bad output tensor")
+        out.view(g.shape).copy_(g)
+        return out
     _lib.require_device(x, smaps, apod, img_index)
     x = _c64(x, "x")
     if tuple(x.shape[-D:]) != tuple(im_size):
@@ -278,7 +286,13 @@ def pad_fft(x: torch.Tensor, im_size, grid_size, smaps: torch.Tensor | None = No
     else:
         q = nx * ns
         shape = (nx, ns, *G) if smaps is not None else (nx, *G)
-    out = torch.empty(shape, dtype=torch.complex64, device=x.device)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.complex64, device=x.device)
+    else:
+        _lib.require_device(out)
+        if out.numel() != math.prod(shape) or out.dtype != torch.complex64 or not out.is_contiguous():
+            raise ValueError("pad_fft: bad output tensor")
+        out = out.view(shape)
     cen = None if center is None else _lib.ints(center)
     check(lib().mra_fft_pad_forward(D, _lib.ints(im_size), _lib.ints(G), cen, ptr(x), npix, ptr(smaps), ns, npix,
                                     ptr(img_index), q, ptr(apod), float(scale), ptr(out),

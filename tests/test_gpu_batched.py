@@ -42,17 +42,15 @@ def test_batched_operators_match_plain(dev):
     probs, ksp = _problems(dev)
     B = BatchedSense(probs, oversamp=1.25, width=4)
     assert len(B.groups) == 3
-    for g in B.groups:
-        X = torch.stack([synth.random_image((g.N, g.N), seed=i) for i in g.idx]).to(dev)
-        Y = g.forward(X)
-        Ys = torch.cat([synth.random_image(ksp[i].shape, seed=i + 7).to(dev).reshape(-1) for i in g.idx])
-        Z = g.adjoint(Ys)
-        for j, i in enumerate(g.idx):
-            A = _plain(probs[i], dev)
-            y = A.forward(X[j])
-            assert rel_l2(Y[g.ksp_off[j]:g.ksp_off[j + 1]], y.reshape(-1)) < 1e-5
-            z = A.adjoint(Ys[g.ksp_off[j]:g.ksp_off[j + 1]].reshape(ksp[i].shape))
-            assert rel_l2(Z[j], z) < 1e-5
+    xs = [synth.random_image(tuple(pr.mps.shape[1:]), seed=i).to(dev) for i, pr in enumerate(probs)]
+    Y = B.forward(B.stack_images(xs))  # one interp launch over all three image sizes
+    Ys = B.stack_ksp([synth.random_image(k.shape, seed=i + 7).to(dev) for i, k in enumerate(ksp)])
+    Z = B.unstack_images(B.adjoint(Ys))  # one gridding launch over all three image sizes
+    for i, pr in enumerate(probs):
+        A = _plain(pr, dev)
+        sl = B.ksp_slice(i)
+        assert rel_l2(Y[sl], A.forward(xs[i]).reshape(-1)) < 1e-5
+        assert rel_l2(Z[i], A.adjoint(Ys[sl].reshape(ksp[i].shape))) < 1e-5
 
 
 def test_batched_cg_matches_loop_of_plain_calls(dev):
