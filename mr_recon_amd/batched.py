@@ -166,24 +166,29 @@ class BatchedSense:
         self.ksp_len = ksp_base
         self.maxC = maxC
         self.geoms = make_geoms(entries, dev)
+        # the normal operator keeps its intermediate k-space in plan (sorted) order, so the
+        # gather stores it and gridding loads it contiguously
+        self.geoms_po = make_geoms([dict(e, ksp_plan_order=True) for e in entries], dev)
         self.items = torch.cat(items).contiguous()
         self.tile_items = torch.cat(tiles).to(torch.int32).contiguous()
         self.tsort = torch.cat(tsort)
         self.perm = torch.cat(perm)
         self.wsort = torch.cat(wsort)
         self.lbase = torch.cat(lbase)
-        self.dcf = None
+        self.dcf = self.dcf_po = None
         if any(d is not None for d in dcfs):
             ms = [m for g in self.groups for m in g.M]
-            self.dcf = torch.cat([(d if d is not None else torch.ones(m, device=dev)).reshape(-1).float()
-                                  for d, m in zip(dcfs, ms)]).contiguous()
+            full = [(d if d is not None else torch.ones(m, device=dev)).reshape(-1).float() for d, m in zip(dcfs, ms)]
+            self.dcf = torch.cat(full).contiguous()
+            self.dcf_po = torch.cat([d[pm.long()] for d, pm in zip(full, perm)]).contiguous()
         self.cc = max(1, min(maxC, L.mra_max_coil_chunk(2, self.P)))
         self.nchunks = -(-maxC // self.cc)
         self.scratch = torch.empty(max(1, scratch), dtype=complex_dtype, device=dev)
         self.gridbuf = torch.empty(max(1, grid_base), dtype=complex_dtype, device=dev)
 
     # -- operators: images Xs (one (nimg, N, N) tensor per group) <-> k-space Y (ksp_len) --
-    def forward(self, Xs: Sequence[torch.Tensor]) -> torch.Tensor:
+    def forward(self, Xs: Sequence[torch.Tensor], plan_order: bool = False) -> torch.Tensor:
+        """``plan_order``: k-space in each problem's sorted (plan) order instead of sample order."""
         st = stream_ptr(self.device)
         for g, X in zip(self.groups, Xs):
             view = g.grid_view(self.gridbuf)
@@ -196,17 +201,18 @@ class BatchedSense:
                                                    ptr(view), st))
                 view.copy_(_fftn(view, 2))
         Y = torch.empty(self.ksp_len, dtype=complex_dtype, device=self.device)
+        geoms = self.geoms_po if plan_order else self.geoms
         check(lib().mra_interp(2, self.P, ptr(self.gridbuf), ptr(Y), ptr(self.wsort), ptr(self.lbase),
-                               ptr(self.perm), ptr(self.geoms), ptr(self.items), self.nitems, self.cc, self.nchunks,
-                               st))
+                               ptr(self.perm), ptr(geoms), ptr(self.items), self.nitems, self.cc, self.nchunks, st))
         return Y
 
-    def adjoint(self, Y: torch.Tensor, scale: float = 1.0) -> list[torch.Tensor]:
+    def adjoint(self, Y: torch.Tensor, scale: float = 1.0, plan_order: bool = False) -> list[torch.Tensor]:
         st = stream_ptr(self.device)
         if Y.numel() != self.ksp_len or Y.dtype != complex_dtype or not Y.is_contiguous():
             raise ValueError(f"adjoint: k-space must be contiguous complex64 with {self.ksp_len} elements")
-        check(lib().mra_grid_partial(2, self.P, ptr(Y), ptr(self.dcf), ptr(self.wsort), ptr(self.lbase),
-                                     ptr(self.perm), ptr(self.geoms), ptr(self.items), self.nitems, self.maxC,
+        geoms, dcf = (self.geoms_po, self.dcf_po) if plan_order else (self.geoms, self.dcf)
+        check(lib().mra_grid_partial(2, self.P, ptr(Y), ptr(dcf), ptr(self.wsort), ptr(self.lbase),
+                                     ptr(self.perm), ptr(geoms), ptr(self.items), self.nitems, self.maxC,
                                      ptr(self.scratch), st))
         for g in self.groups:
             check(lib().mra_grid_reduce(2, self.P, ptr(self.gridbuf), ptr(self.scratch), ptr(self.geoms), g.prob0,
@@ -227,7 +233,7 @@ class BatchedSense:
         return Xs
 
     def normal(self, Xs: Sequence[torch.Tensor], scale: float = 1.0) -> list[torch.Tensor]:
-        return self.adjoint(self.forward(Xs), scale)
+        return self.adjoint(self.forward(Xs, plan_order=True), scale, plan_order=True)
 
     # stacking helpers -------------------------------------------------------------------
     def ksp_slice(self, i: int) -> slice:

@@ -53,6 +53,20 @@ def test_batched_operators_match_plain(dev):
         assert rel_l2(Z[i], A.adjoint(Ys[sl].reshape(ksp[i].shape))) < 1e-5
 
 
+def test_batched_normal_keeps_ksp_in_plan_order(dev):
+    """The batched normal operator keeps k-space in plan order: equal to adjoint(forward)."""
+    from mr_recon_amd.batched import BatchedSense
+
+    probs, _ = _problems(dev, seed=5)
+    B = BatchedSense(probs, oversamp=1.25, width=4)
+    xs = [synth.random_image(tuple(pr.mps.shape[1:]), seed=20 + i).to(dev) for i, pr in enumerate(probs)]
+    Xs = B.stack_images(xs)
+    got = B.normal(Xs, 0.5)
+    ref = B.adjoint(B.forward(Xs), 0.5)
+    for a, b in zip(got, ref):
+        assert torch.equal(a, b)
+
+
 def test_batched_cg_matches_loop_of_plain_calls(dev):
     from mr_recon_amd.batched import BatchedSense
     from mr_recon_amd.recons import CG_SENSE_recon

@@ -55,6 +55,18 @@ def test_sense_toeplitz_normal(dev, case):
     assert rel_l2(A.normal(x.to(dev)), R.normal(x)) < TOL
 
 
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_sense_normal_keeps_ksp_in_plan_order(dev, case):
+    """Without Toeplitz kernels the normal operator keeps its k-space in plan order (stored
+    and loaded contiguously): the same sums in the same order as adjoint(forward(x))."""
+    A, R, im = _ops(case, dev)
+    x = synth.random_image(im, seed=case + 5).to(dev)
+    got = A.normal(x)
+    ref = A.adjoint(A.forward(x))
+    assert torch.equal(got, ref)
+    assert rel_l2(got, R.normal(x.cpu())) < TOL
+
+
 def test_sense_normal_is_hermitian(dev):
     A, _, im = _ops(0, dev, toeplitz=False)
     x = synth.random_image(im, seed=1).to(dev)
