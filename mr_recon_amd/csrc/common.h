@@ -39,8 +39,9 @@ struct Geom {
   long long scratch_off;  // float2 offset of this problem's gridding partials
   int tile_items_off;     // offset of this problem's per-tile item table
   int dcf_plan_order;     // 1: the gridding dcf is indexed by sorted (plan) position, 0: by sample
-  int ksp_plan_order;     // 1: k-space (interp output, gridding input) and dcf are in sorted order
-                          //    (interp and gridding only; the trajectory gradient ignores it)
+  int ksp_plan_order;     // 1: k-space (interp output, gridding input, the trajectory gradient's
+                          //    coefficients) and dcf are in sorted order; the trajectory
+                          //    gradient's output stays in sample order
 };
 
 // Position of sorted sample gs in the problem's sample-indexed vectors (k-space, dcf):

@@ -528,7 +528,7 @@ __device__ __forceinline__ void patch_planes(int bi, int (&off)[P]) {
 }
 
 // Old subgrid values at the patch of the run whose word is w.
-template <int D, int P, int T>
+template <int D, int P, int T, int PE = P>
 __device__ __forceinline__ void read_run(int w, const bool (&rok)[RowLanes<D, P, T>::NR],
                                          const int (&roff)[RowLanes<D, P, T>::NR], const float2* sgrid,
                                          float2 (&pre)[RowLanes<D, P, T>::CPL][RowLanes<D, P, T>::NR][P]) {
@@ -541,7 +541,7 @@ __device__ __forceinline__ void read_run(int w, const bool (&rok)[RowLanes<D, P,
     for (int q = 0; q < RL::NR; ++q) {
       const float2* pt = sgrid + u * RL::NS * RL::SSTR + word_rest(w) + (rok[q] ? roff[q] : 0);
 #pragma unroll
-      for (int i = 0; i < P; ++i) pre[u][q][i] = pt[off[i]];
+      for (int i = 0; i < PE; ++i) pre[u][q][i] = pt[off[i]];
     }
 }
 
@@ -575,8 +575,9 @@ __device__ __forceinline__ void retire_planes(int lo, int hi, int nc, float2* ld
 
 // Accumulate stage sample t; at the end of its run (bit 30 of its word) write old + sum
 // back and clear the sums; then, before the next run's old values are read, store the
-// planes the next run has moved past.
-template <int D, int P, int T>
+// planes the next run has moved past. PE = P - 1 when no sample of the item reaches the
+// last patch plane (its dim-0 weight there is exactly 0): that plane is left untouched.
+template <int D, int P, int T, int PE = P>
 __device__ __forceinline__ void accumulate_sample(
     const SampleView<D, P, RowLanes<D, P, T>::NR, RowLanes<D, P, T>::CPL>& sv, int t, int n, int pub,
     const bool (&rok)[RowLanes<D, P, T>::CPL][RowLanes<D, P, T>::NR], const int (&roff)[RowLanes<D, P, T>::NR],
@@ -594,7 +595,7 @@ __device__ __forceinline__ void accumulate_sample(
     for (int u = 0; u < RL::CPL; ++u) {
       const float vx = sv.v[u].x * wr, vy = sv.v[u].y * wr;
 #pragma unroll
-      for (int i = 0; i < P; ++i) {
+      for (int i = 0; i < PE; ++i) {
         acc[u][q][i].x += sv.wx[i] * vx;
         acc[u][q][i].y += sv.wx[i] * vy;
       }
@@ -610,11 +611,11 @@ __device__ __forceinline__ void accumulate_sample(
         if (rok[u][q]) {
           float2* pt = sgrid + u * RL::NS * RL::SSTR + word_rest(raw) + roff[q];
 #pragma unroll
-          for (int i = 0; i < P; ++i)
+          for (int i = 0; i < PE; ++i)
             pt[off[i]] = make_float2(pre[u][q][i].x + acc[u][q][i].x, pre[u][q][i].y + acc[u][q][i].y);
         }
 #pragma unroll
-        for (int i = 0; i < P; ++i) acc[u][q][i] = make_float2(0.f, 0.f);
+        for (int i = 0; i < PE; ++i) acc[u][q][i] = make_float2(0.f, 0.f);
       }
     if (t + 1 < n) {
       const int nw = __builtin_amdgcn_readlane(pub, t + 1);
@@ -623,35 +624,21 @@ __device__ __forceinline__ void accumulate_sample(
         retire_planes<D, P, T>(lo, bn, nc, lds2, dst);
         lo = bn;
       }
-      read_run<D, P, T>(nw, rok[0], roff, sgrid, pre);
+      read_run<D, P, T, PE>(nw, rok[0], roff, sgrid, pre);
     }
   }
 }
 
-template <int D, int P, int T>
-__global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restrict__ ksp,
-                                                          const float* __restrict__ dcf,
-                                                          const float* __restrict__ wsort,
-                                                          const int* __restrict__ lbase,
-                                                          const int* __restrict__ perm,
-                                                          const Geom* __restrict__ geoms,
-                                                          const Item* __restrict__ items, int nitems,
-                                                          int nchunks, float2* __restrict__ scratch) {
+// One (item, coil chunk) of grid_partial_kernel with PE rows per patch (P, or P - 1 when no
+// sample of the item reaches the last patch plane).
+template <int D, int P, int T, int PE>
+__device__ __forceinline__ void grid_partial_item(const float2* __restrict__ ksp, const float* __restrict__ dcf,
+                                                  const float* __restrict__ wsort, const int* __restrict__ lbase,
+                                                  const int* __restrict__ perm, const Geom& g, const Item& it,
+                                                  int iidx, int c0, int nc, float2* __restrict__ scratch,
+                                                  float2* lds2, float* rec, float* wl, float2* vst) {
   using RL = RowLanes<D, P, T>;
   constexpr int SUB = RL::SUB;
-  extern __shared__ float2 lds2[];
-  __shared__ float rec[RL::WS * RL::RECW];
-  __shared__ float wl[RL::NWL * RL::WS];
-  __shared__ float2 vst[RL::NC * RL::WS];
-  int iidx, cc;
-  item_of_block(blockIdx.x, nchunks, iidx, cc);
-  if (iidx >= nitems) return;
-  const Item it = items[iidx];
-  if (it.w <= 0 || it.y < 0) return;  // non-finite samples contribute nothing
-  const Geom g = geoms[it.x];
-  const int c0 = cc * RL::NC;
-  if (c0 >= g.nbatch) return;
-  const int nc = min(RL::NC, g.nbatch - c0);
   const int lane = threadIdx.x;
   const int cnt = it.w;
   const long long gs0 = g.samp_off + it.z;
@@ -746,7 +733,7 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
         retire_planes<D, P, T>(lo, b0, nc, lds2, dst);
         lo = b0;
       }
-      read_run<D, P, T>(w0, rok[0], roff, sgrid, pre);
+      read_run<D, P, T, PE>(w0, rok[0], roff, sgrid, pre);
     }
     // ring of LA register sets, loaded LA-2..LA-1 samples ahead; the stage arrays are
     // padded so loads past the last sample stay in bounds (their values are unused)
@@ -757,7 +744,7 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
 #pragma unroll
       for (int u = 0; u < RL::LA; ++u) {
         if (t + u < n)
-          accumulate_sample<D, P, T>(ring[u], t + u, n, pub, rok, roff, sgrid, lds2, dst, nc, lo, acc, pre);
+          accumulate_sample<D, P, T, PE>(ring[u], t + u, n, pub, rok, roff, sgrid, lds2, dst, nc, lo, acc, pre);
         read_view<D, P, T>(ring[(u + RL::LA - 1) % RL::LA], t + u + RL::LA - 1, rec, wl, vmine, ridx);
       }
     }
@@ -765,6 +752,44 @@ __global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restri
   }
   // the remaining planes (held or never reached) are final
   retire_planes<D, P, T>(lo, SUB, nc, lds2, dst);
+}
+
+
+template <int D, int P, int T>
+__global__ __launch_bounds__(64) void grid_partial_kernel(const float2* __restrict__ ksp,
+                                                          const float* __restrict__ dcf,
+                                                          const float* __restrict__ wsort,
+                                                          const int* __restrict__ lbase,
+                                                          const int* __restrict__ perm,
+                                                          const Geom* __restrict__ geoms,
+                                                          const Item* __restrict__ items, int nitems,
+                                                          int nchunks, float2* __restrict__ scratch) {
+  using RL = RowLanes<D, P, T>;
+  extern __shared__ float2 lds2[];
+  __shared__ float rec[RL::WS * RL::RECW];
+  __shared__ float wl[RL::NWL * RL::WS];
+  __shared__ float2 vst[RL::NC * RL::WS];
+  int iidx, cc;
+  item_of_block(blockIdx.x, nchunks, iidx, cc);
+  if (iidx >= nitems) return;
+  const Item it = items[iidx];
+  if (it.w <= 0 || it.y < 0) return;  // non-finite samples contribute nothing
+  const Geom g = geoms[it.x];
+  const int c0 = cc * RL::NC;
+  if (c0 >= g.nbatch) return;
+  const int nc = min(RL::NC, g.nbatch - c0);
+  // does a sample of the item reach the last patch plane (dim-0 weight P-1 nonzero)?
+  // (one pass over those weights; the item's coil chunks run back to back on one XCD)
+  int edge = 0;
+  if constexpr (P >= 3) {
+    const float* w0 = wsort + (g.samp_off + it.z) * (D * P);
+    for (int s = threadIdx.x; s < it.w; s += 64) edge |= w0[(long long)s * (D * P) + P - 1] != 0.0f;
+  }
+  if (P < 3 || __any(edge))
+    grid_partial_item<D, P, T, P>(ksp, dcf, wsort, lbase, perm, g, it, iidx, c0, nc, scratch, lds2, rec, wl, vst);
+  else
+    grid_partial_item<D, P, T, (P >= 3 ? P - 1 : P)>(ksp, dcf, wsort, lbase, perm, g, it, iidx, c0, nc, scratch,
+                                                      lds2, rec, wl, vst);
 }
 
 }  // namespace mra
@@ -799,6 +824,54 @@ __device__ __forceinline__ void reduce_pairs_2d(float2* __restrict__ grid, const
   const int gc[2] = {tc[0] * T + pr / (T / 2), tc[1] * T + 2 * (pr % (T / 2))};
   const bool inside = gc[0] < g.G[0] && gc[1] < g.G[1];
   float4 acc[2] = {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+  if (ncand[0] <= 2 && ncand[1] <= 2) {
+    // at most 2 x 2 covering tiles: item ranges first, then the first item of every tile
+    // in one unrolled batch of loads, then any further items (as in reduce_pairs_3d)
+    const float2* sp[4];
+    int nxt[4], end[4], ex[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int k0 = c >> 1, k1 = c & 1;
+      const bool valid = k0 < ncand[0] && k1 < ncand[1];
+      const int c0 = cand[0][valid ? k0 : 0], c1 = cand[1][valid ? k1 : 0];
+      const int l0 = pmod(gc[0] - c0 * T, g.G[0]);
+      const int l1 = pmod(gc[1] - c1 * T, g.G[1]);  // even, so l1 + 1 < SUB when l1 < SUB
+      const bool ok = valid && inside && l0 < TS::SUB && l1 < TS::SUB;
+      ex[c] = l0 * TS::SUB + l1;
+      const int stile = c0 * g.ntile[1] + c1;
+      const int i0 = ok ? titems[stile] : 0, i1 = ok ? titems[stile + 1] : 0;
+      sp[c] = i1 > i0 ? sbase + ((long long)(i0 - g.item_base) * g.nbatch + b0) * TS::SUBN + ex[c] : nullptr;
+      nxt[c] = i0 + 1;
+      end[c] = i1;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (sp[c]) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+          if (2 * h + u < nb) {
+            const float4 v = *(const float4*)(sp[c] + (long long)(2 * h + u) * TS::SUBN);
+            acc[u].x += v.x;
+            acc[u].y += v.y;
+            acc[u].z += v.z;
+            acc[u].w += v.w;
+          }
+      }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      for (int it = nxt[c]; it < end[c]; ++it) {
+        const float2* slot = sbase + ((long long)(it - g.item_base) * g.nbatch + b0) * TS::SUBN + ex[c];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+          if (2 * h + u < nb) {
+            const float4 v = *(const float4*)(slot + (long long)(2 * h + u) * TS::SUBN);
+            acc[u].x += v.x;
+            acc[u].y += v.y;
+            acc[u].z += v.z;
+            acc[u].w += v.w;
+          }
+      }
+  } else
   for (int k0 = 0; k0 < ncand[0]; ++k0)
     for (int k1 = 0; k1 < ncand[1]; ++k1) {
       const int c0 = cand[0][k0], c1 = cand[1][k1];
@@ -1169,7 +1242,7 @@ __global__ __launch_bounds__(kThreads) void interp_grad_kernel(const float2* __r
 
   constexpr int SUB = T + P - 1;
   float w[K][D][P], dw[K][D][P];
-  int base[K], oidx[K];
+  int base[K], oidx[K], aidx[K];  // output slot (sample order); slot of the coefficients a
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int s = s0 + tid + k * kThreads;
@@ -1185,6 +1258,7 @@ __global__ __launch_bounds__(kThreads) void interp_grad_kernel(const float2* __r
       }
       base[k] = bs;
       oidx[k] = perm[g.samp_off + s];
+      aidx[k] = g.ksp_plan_order ? s : oidx[k];
     } else {
 #pragma unroll
       for (int d = 0; d < D; ++d)
@@ -1192,6 +1266,7 @@ __global__ __launch_bounds__(kThreads) void interp_grad_kernel(const float2* __r
         for (int j = 0; j < P; ++j) w[k][d][j] = dw[k][d][j] = 0.f;
       base[k] = 0;
       oidx[k] = -1;
+      aidx[k] = 0;
     }
   }
 
@@ -1226,7 +1301,7 @@ __global__ __launch_bounds__(kThreads) void interp_grad_kernel(const float2* __r
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       if (oidx[k] < 0) continue;
-      const float2 av = asrc[(long long)b * g.ksp_stride + oidx[k]];
+      const float2 av = asrc[(long long)b * g.ksp_stride + aidx[k]];
       const float2* sb = lds2 + cur + base[k];
       float2 sd[D];
 #pragma unroll

@@ -50,6 +50,8 @@ class _Ctx:
         self.mps = None if mps is None else mps.to(complex_dtype).contiguous()
         self.C = 1 if mps is None else mps.shape[0]
         self.dcf = None if dcf is None else dcf.to(torch.float32).reshape(-1).contiguous()
+        # plan (sorted) order copy: the normal operator keeps its k-space in that order
+        self.dcf_po = None if self.dcf is None else self.dcf[self.plan.perm.long()].contiguous()
         self.apod = nufft.apod_table(trj.device)
         self.s_fwd = 1.0 / math.sqrt(math.prod(self.im_size))
         self.s_interp = 1.0 / nufft.width ** self.d
@@ -64,18 +66,24 @@ class _Ctx:
         return g.reshape(self.C, *self.G)
 
     def forward(self, x):
-        out = torch.empty((self.C, self.plan.npts), dtype=complex_dtype, device=x.device)
-        return ops.interp(self.grids(x), self.plan, out_scale=self.s_interp, out=out)
+        return self.gather(self.grids(x))
 
-    def adjoint(self, y, dcf):
+    def gather(self, grids, plan_order=False):
+        """Interpolation of the oversampled grids: (C, M) in sample order, or in plan order."""
+        out = torch.empty((self.C, self.plan.npts), dtype=complex_dtype, device=grids.device)
+        geo = self.plan.geom(self.C, self.plan.npts, self.s_interp, ksp_plan_order=True) if plan_order else None
+        return ops.interp(grids, self.plan, out_scale=self.s_interp, out=out, geom=geo)
+
+    def adjoint(self, y, dcf, plan_order=False):
         g = torch.empty((self.C, *self.G), dtype=complex_dtype, device=y.device)
-        ops.gridding(y.to(complex_dtype).reshape(self.C, -1), self.plan, g, dcf=dcf)
+        ops.gridding(y.to(complex_dtype).reshape(self.C, -1), self.plan, g, dcf=dcf, plan_order=plan_order)
         return ops.ifft_crop(g, self.im_size, self.G, 1, smaps=self.mps, apod=self.apod, scale=self.s_adj)[0]
 
-    def dk(self, grids, a):
-        """dL/dk (M, d) from the KB-derivative gather of ``grids`` against coefficients ``a``."""
+    def dk(self, grids, a, plan_order=False):
+        """dL/dk (M, d) from the KB-derivative gather of ``grids`` against coefficients ``a``
+        (sample order, or plan order)."""
         gt = ops.interp_grad(grids, a.to(complex_dtype).reshape(self.C, -1).contiguous(), self.plan,
-                             out_scale=self.s_interp)
+                             out_scale=self.s_interp, a_plan_order=plan_order)
         return (gt * self.dt_dk).reshape(*self.trj_shape, self.d)
 
 
@@ -134,6 +142,45 @@ class SenseAdjointFn(Function):
         return gy, gt, None, None, None
 
 
+class SenseNormalFn(Function):
+    """(x (*im), trj) -> A^H D A x in one function, the k-space in between kept in plan
+    (sorted) order so the gather stores it and gridding loads it contiguously. Backward
+    (D real, A^H D A self-adjoint), with g the output cotangent and G_g = FFT(pad(S g)):
+        gx = A^H D A g
+        gt = dk(G_g, D A x) + dk(G_x, D A g)      (the adjoint's and the forward's k-terms)
+    G_g is computed once and serves both the gather D A g and the first k-term; A x is
+    saved from the forward."""
+
+    @staticmethod
+    def forward(ctx, x, trj, nufft, mps, dcf):
+        c = _Ctx(nufft, trj, mps, dcf)
+        y = c.gather(c.grids(x.detach()), plan_order=True)
+        out = c.adjoint(y, c.dcf_po, plan_order=True)
+        ctx.c = c
+        ctx.save_for_backward(x.detach(), y)
+        return out
+
+    @staticmethod
+    def backward(ctx, gout):
+        c = ctx.c
+        x, y = ctx.saved_tensors
+        gout = gout.to(complex_dtype).contiguous()
+        gx = gt = None
+        need_x, need_t = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
+        if need_x or need_t:
+            Gg = c.grids(gout)
+            ag = c.gather(Gg, plan_order=True)  # A g, plan order
+            dag = ag * c.dcf_po if c.dcf_po is not None else ag
+            if need_x:
+                gx = c.adjoint(ag, c.dcf_po, plan_order=True).reshape(x.shape)
+                if not torch.is_complex(x):
+                    gx = gx.real
+            if need_t:
+                dy = y * c.dcf_po if c.dcf_po is not None else y
+                gt = c.dk(Gg, dy, plan_order=True) + c.dk(c.grids(x), dag, plan_order=True)
+        return gx, gt, None, None, None
+
+
 def nufft_forward(img: torch.Tensor, trj: torch.Tensor, nufft: hip_nufft) -> torch.Tensor:
     """Differentiable single-image NUFFT: img (*im_size), trj (*trj, d) -> ksp (*trj)."""
     return SenseForwardFn.apply(img, trj, nufft, None)[0]
@@ -153,8 +200,8 @@ def sense_adjoint(y, trj, mps, nufft: hip_nufft, dcf=None):
 
 
 def sense_normal(x, trj, mps, nufft: hip_nufft, dcf=None):
-    """A^H D A x, differentiable in x and trj."""
-    return sense_adjoint(sense_forward(x, trj, mps, nufft), trj, mps, nufft, dcf)
+    """A^H D A x, differentiable in x and trj (one autograd function, k-space in plan order)."""
+    return SenseNormalFn.apply(x, trj, nufft, mps, dcf)
 
 
 class _DotRe64(Function):
@@ -219,5 +266,5 @@ def unrolled_cg_sense(ksp, trj, mps, nufft: hip_nufft, num_iters: int = 10, lamd
     return x
 
 
-__all__ = ["SenseForwardFn", "SenseAdjointFn", "nufft_forward", "nufft_adjoint", "sense_forward",
+__all__ = ["SenseForwardFn", "SenseAdjointFn", "SenseNormalFn", "nufft_forward", "nufft_adjoint", "sense_forward",
            "sense_adjoint", "sense_normal", "unrolled_cg_sense", "dot_re64"]

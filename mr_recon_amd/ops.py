@@ -47,16 +47,19 @@ def interp(grid: torch.Tensor, plan: SamplePlan, out_scale: float = 1.0, out: to
 
 
 def gridding(ksp: torch.Tensor, plan: SamplePlan, out: torch.Tensor, dcf: torch.Tensor | None = None,
-             out_scale: float = 1.0, accumulate: bool = False, scratch: torch.Tensor | None = None) -> torch.Tensor:
+             out_scale: float = 1.0, accumulate: bool = False, scratch: torch.Tensor | None = None,
+             plan_order: bool = False) -> torch.Tensor:
     """Adjoint of ``interp``: samples (B, M) complex64 -> grid ``out`` (B, *G).
 
     Every point of ``out`` is written (``accumulate=True`` adds to its contents
     instead), so ``out`` needs no zeroing. ``dcf`` (M,) float32 multiplies each
     sample first. ``scratch`` (optional) holds the per-item partial subgrids,
-    ``gridding_scratch_elems(plan, B)`` complex64 elements.
+    ``gridding_scratch_elems(plan, B)`` complex64 elements. ``plan_order``: ``ksp``
+    and ``dcf`` are in sorted (plan) order.
     """
     B = _grid_batch(plan, out)
-    scratch = gridding_partial(ksp, plan, B, dcf=dcf, out_scale=out_scale, scratch=scratch)
+    scratch = gridding_partial(ksp, plan, B, dcf=dcf, out_scale=out_scale, scratch=scratch,
+                               dcf_plan_order=plan_order, ksp_plan_order=plan_order)
     return gridding_reduce(scratch, plan, out, accumulate=accumulate)
 
 
@@ -141,10 +144,14 @@ def gridding_scratch_elems(plan: SamplePlan, nbatch: int) -> int:
     return max(1, plan.nitems * nbatch * plan.subgrid)
 
 
-def interp_grad(grid: torch.Tensor, a: torch.Tensor, plan: SamplePlan, out_scale: float = 1.0) -> torch.Tensor:
-    """d/dt of Re<a, interp(grid, t)> per sample: returns (M, D) float32 in grid units.
+def interp_grad(grid: torch.Tensor, a: torch.Tensor, plan: SamplePlan, out_scale: float = 1.0,
+                a_plan_order: bool = False) -> torch.Tensor:
+    """d/dt of Re<a, interp(grid, t)> per sample: returns (M, D) float32 in grid units,
+    in sample order.
 
     gt[m, d] = sum_b Re( conj(a[b, m]) * sum_p dW_p/dt_d * grid[b, g_p] )
+
+    ``a_plan_order``: ``a`` is given in sorted (plan) order.
     """
     _lib.require_device(grid, a)
     if len(set(plan.betas)) > 1:
@@ -156,7 +163,7 @@ def interp_grad(grid: torch.Tensor, a: torch.Tensor, plan: SamplePlan, out_scale
     if a.numel() != B * plan.npts:
         raise ValueError("interp_grad: a must hold B*M samples")
     gt = torch.empty((plan.npts, D), dtype=torch.float32, device=grid.device)
-    g = plan.geom(B, plan.npts, out_scale)
+    g = plan.geom(B, plan.npts, out_scale, ksp_plan_order=a_plan_order)
     check(lib().mra_interp_grad(D, plan.P, ptr(grid), ptr(a), ptr(gt), ptr(plan.tsort), ptr(plan.perm), ptr(g),
                                 ptr(plan.items), plan.nitems, plan.S, stream_ptr(grid.device)))
     return gt

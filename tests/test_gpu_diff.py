@@ -174,3 +174,39 @@ def test_unrolled_cg_gradients_checkpointed(dev):
     fd = (oracle_unrolled(k64 + h * v) - oracle_unrolled(k64 - h * v)) / (2 * h)
     an = float(torch.sum(g1.cpu().to(torch.float64) * v))
     assert abs(an - fd) / abs(fd) < 5e-3, (an, fd)
+
+
+@pytest.mark.parametrize("dim,use_dcf", [(2, True), (2, False), (3, True)])
+def test_normal_function_matches_composition(dev, dim, use_dcf):
+    """SenseNormalFn (one autograd function, k-space kept in plan order) against
+    sense_adjoint(sense_forward(x)): output, dL/dx and dL/dk for a loss that uses both."""
+    from mr_recon_amd.diff import SenseNormalFn, sense_adjoint, sense_forward
+    from mr_recon_amd.fourier import hip_nufft
+
+    if dim == 2:
+        trj, mps, x, _ = _problem(seed=11)
+    else:
+        im = (14, 12, 16)
+        trj = synth.kooshball_3d(40, 24, 14).reshape(-1, 3) + 0.2 * (torch.rand(960, 3) - 0.5)
+        mps = synth.coil_maps(3, im, seed=3)
+        x = synth.random_image(im, seed=4)
+    dcf = (torch.rand(trj.shape[0], generator=torch.Generator().manual_seed(2)) + 0.5).to(dev) if use_dcf else None
+    nf = hip_nufft(tuple(x.shape), oversamp=1.25, width=4)
+    target = synth.random_image(tuple(x.shape), seed=9).to(dev)
+
+    def run(fused):
+        xd = x.to(dev).requires_grad_(True)
+        kd = trj.to(dev).requires_grad_(True)
+        if fused:
+            out = SenseNormalFn.apply(xd, kd, nf, mps.to(dev), dcf)
+        else:
+            out = sense_adjoint(sense_forward(xd, kd, mps.to(dev), nf), kd, mps.to(dev), nf, dcf)
+        loss = torch.sum(torch.abs(out - target) ** 2)
+        loss.backward()
+        return out.detach(), xd.grad, kd.grad
+
+    o1, gx1, gk1 = run(True)
+    o2, gx2, gk2 = run(False)
+    assert rel_l2(o1, o2) < 1e-6
+    assert rel_l2(gx1, gx2) < 1e-6
+    assert rel_l2(gk1, gk2) < 1e-6

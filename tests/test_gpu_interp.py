@@ -108,6 +108,24 @@ def test_ungrid_full_and_short_patches(dev, D, G, width, B, frac):
     assert rel_l2(out, ref) < TOL
 
 
+@pytest.mark.parametrize("D,G,width,B", [(2, 48, 4.0, 3), (2, 40, 6.0, 9), (2, 40, 5.0, 2), (1, 96, 4.0, 2),
+                                         (3, 20, 8.0, 2)])
+@pytest.mark.parametrize("frac", [0.0, 0.01, 1.0])
+def test_grid_full_and_short_patches(dev, D, G, width, B, frac):
+    """Lane gridding: an item whose samples all miss the last patch plane (dim-0 weight 0)
+    accumulates P-1 planes per sample; one with a sample on the lattice keeps P."""
+    from mr_recon_amd.hip_interp import grid
+
+    g = torch.Generator().manual_seed(D * 1000 + G + int(10 * width) + int(100 * frac))
+    n = 3000
+    k = torch.rand((n, D), generator=g) * G
+    on = torch.rand(n, generator=g) < frac
+    k[on, 0] = torch.floor(k[on, 0]) + (width / 2 - math.floor(width / 2))
+    vals = torch.randn((B, n), dtype=torch.complex64, generator=g)
+    out = grid(vals.to(dev), k.to(dev), (G,) * D, width, kernel_params={"beta": 6.3})
+    assert rel_l2(out, interp_ref.grid(vals, k, (G,) * D, width, beta=6.3)) < TOL
+
+
 @pytest.mark.parametrize("D,G,width,B", [(1, 128, 4.0, 2), (2, 48, 4.0, 2), (2, 64, 6.0, 3), (3, 20, 4.0, 2),
                                          (3, 16, 6.0, 1), (2, 32, 3.0, 1)])
 def test_grid_matches_oracle(dev, D, G, width, B):
