@@ -741,12 +741,12 @@ def main():
         # the other two halves of the metric: batched and trajectory-gradient forms against
         # the plain per-problem path on this GPU (each reports its agreement with it)
         t0 = time.time()
-        b = measure_batched(steps=8, warmup=1)
+        b = measure_batched(steps=8, warmup=3)
         rec["batched"] = {k: b[k] for k in ("value", "unit", "ms_per_step", "plain_loop", "speedup_vs_plain",
                                             "max_rel_l2_vs_plain", "total_samples")}
         rec["batched"]["workload"] = b["config"]["workload"]
         torch.cuda.empty_cache()
-        g = measure_grad(steps=8, warmup=1)
+        g = measure_grad(steps=8, warmup=3)
         rec["grad"] = {k: g[k] for k in ("value", "unit", "ms_per_step", "plain_path", "speedup_vs_plain",
                                          "agreement")}
         rec["grad"]["workload"] = g["config"]["workload"]
