@@ -408,6 +408,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, kIn
 #ifndef MRA_GRID_CPL
 #define MRA_GRID_CPL 1
 #endif
+#ifndef MRA_GRID_NSMAX
+#define MRA_GRID_NSMAX 8  // most coil slots per wave
+#endif
 
 // LDS padding (row pitch of the last dim += E in 3D, coil-slot stride += SP), from a
 // bank model of the run read-modify-write (64 banks, 8-byte lanes) per (D, P).
@@ -430,7 +433,7 @@ struct RowLanes {
   static constexpr int SUBN = TileShape<D, P, T>::SUBN;
   static constexpr int NR = (PR + 63) / 64;                        // rest positions per lane
   static constexpr int LPS = PR < 64 ? PR : 64;                    // lanes per coil slot
-  static constexpr int NS = (64 / LPS) < 8 ? (64 / LPS) : 8;       // lane groups per wave
+  static constexpr int NS = (64 / LPS) < MRA_GRID_NSMAX ? (64 / LPS) : MRA_GRID_NSMAX;  // lane groups per wave
   static constexpr int CPL = MRA_GRID_CPL;                         // coils per lane
   static constexpr int NC = NS * CPL;                              // coils per wave (lane group cs
                                                                    // owns coils cs + u*NS, u < CPL)

@@ -12,7 +12,8 @@ application is one fused pipeline over all coils:
 
 Coils are processed together rather than one at a time (the reference default
 ``coil_batch_size=1``): a 32-coil 160^3 grid is 1 GiB, small against HBM; coil
-chunks are only formed when the grids would exceed ``MAX_GRID_BYTES``. Other
+chunks are only formed when the grids and their gridding scratch would exceed
+``MAX_GRID_BYTES``. Other
 NUFFTs, or spatial/temporal imperfection functions with more than one segment,
 take the reference's generic loop through ``nufft.forward``/``adjoint``.
 """
@@ -33,7 +34,7 @@ from .pad import PadLast
 from .plan import SamplePlan
 from .utils import batch_iterator
 
-MAX_GRID_BYTES = 32 << 30  # coil chunking threshold for the fused path
+MAX_GRID_BYTES = 32 << 30  # coil chunking threshold for the fused path (grids + gridding scratch)
 
 
 @dataclass
@@ -302,7 +303,10 @@ class sense_linop(linop):
                 self.toep_unshifted = toeplitz_unshifted(self.toep_kerns)
             # dcf in plan order: gridding reads it contiguously instead of through perm
             self._dcf_plan = self.dcf.reshape(-1)[self.plan.perm.long()].contiguous()
-            per_coil = 8 * max(math.prod(self.os_shape), math.prod(self.toep_kerns.shape[-d:]) if use_toeplitz else 0)
+            # per coil: its oversampled (or PSF-sized) grid and its gridding scratch (one
+            # subgrid per work item, 2.2x the grid for the 3D kooshball at 128^3)
+            per_coil = 8 * (max(math.prod(self.os_shape), math.prod(self.toep_kerns.shape[-d:]) if use_toeplitz else 0)
+                            + self.plan.nitems * self.plan.subgrid)
             self.coil_chunk = max(1, min(ncoils, MAX_GRID_BYTES // per_coil))
 
         self.timers = None  # {stage: [(start_event, end_event), ...]} when profiling

@@ -1,5 +1,7 @@
 """GPU parity: fused SENSE linop (forward / adjoint / Toeplitz normal), CG and CG-SENSE vs the oracle."""
 
+import math
+
 import pytest
 import torch
 
@@ -168,3 +170,24 @@ def test_sense_default_nufft_is_sigpy_nufft(dev):
     R = SenseRef(trj, mps, nufft=SigpyNufftRef(im))
     x = synth.random_image(im, seed=7)
     assert rel_l2(A.forward(x.to(dev)), R.forward(x)) < TOL
+
+
+@pytest.mark.parametrize("case,toeplitz", [(0, False), (0, True), (2, False)])
+def test_sense_coil_chunks_match_one_chunk(dev, monkeypatch, case, toeplitz):
+    """Coil chunks (formed when the grids and their gridding scratch exceed the threshold):
+    forward, adjoint and normal equal the one-chunk operator; the adjoint and normal add the
+    chunks' images into one output."""
+    from mr_recon_amd import linops
+
+    A1, _, im = _ops(case, dev, toeplitz=toeplitz)
+    assert A1.coil_chunk == A1.mps.shape[0]
+    per_coil = 8 * (max(math.prod(A1.os_shape), math.prod(A1.toep_kerns.shape[-len(im):]) if toeplitz else 0)
+                    + A1.plan.nitems * A1.plan.subgrid)
+    monkeypatch.setattr(linops, "MAX_GRID_BYTES", 3 * per_coil)
+    A3, _, _ = _ops(case, dev, toeplitz=toeplitz)
+    assert A3.coil_chunk == 3 < A3.mps.shape[0]
+    x = synth.random_image(im, seed=case + 30).to(dev)
+    assert torch.equal(A3.forward(x), A1.forward(x))
+    k = synth.random_image(tuple(A1.forward(x).shape), seed=case + 31).to(dev)
+    assert rel_l2(A3.adjoint(k), A1.adjoint(k)) < 1e-6
+    assert rel_l2(A3.normal(x), A1.normal(x)) < 1e-6
