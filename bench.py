@@ -201,14 +201,17 @@ def measure_batched(steps, warmup, samples_per_item="auto"):
         spec.append((N, C, trj.shape[0]))
     kw = dict(num_iters=10, lamda_l2=1e-3, max_eigen=1.0)
     t0 = time.time()
-    B = BatchedSense(probs, oversamp=1.25, width=4, samples_per_item=samples_per_item)
+    B = BatchedSense(probs, oversamp=1.25, width=4, samples_per_item=samples_per_item, tile="auto")
     torch.cuda.synchronize()
-    log(f"batched operators built in {time.time() - t0:.1f}s ({len(B.groups)} groups)")
-    # the plain path gets the same (autotuned) work-item size per problem, so both paths
-    # sum in the same order and their reconstructions can be compared bit for bit
+    log(f"batched operators built and tile edge tuned in {time.time() - t0:.1f}s ({len(B.groups)} groups, "
+        f"tile {B.tile}; ms per normal operator by tile: "
+        + ", ".join(f"{t}: {v:.2f}" for t, v in (B.tile_times or {}).items()) + ")")
+    # the plain path gets the same (autotuned) tile and work-item size per problem, so both
+    # paths sum in the same order and their reconstructions can be compared bit for bit
     S_of = {i: plan.S for g in B.groups for i, plan in zip(g.idx, g.plans)}
     t0 = time.time()
-    plain_ops = [sense_linop(p.trj, p.mps, nufft=hip_nufft((n, n), oversamp=1.25, width=4, samples_per_item=S_of[i]))
+    plain_ops = [sense_linop(p.trj, p.mps, nufft=hip_nufft((n, n), oversamp=1.25, width=4, samples_per_item=S_of[i],
+                                                           tile=B.tile))
                  for i, (p, (n, _, _)) in enumerate(zip(probs, spec))]
     torch.cuda.synchronize()
     log(f"plain operators built in {time.time() - t0:.1f}s")
@@ -239,7 +242,8 @@ def measure_batched(steps, warmup, samples_per_item="auto"):
         "dtype": "c64", "data": "synthetic (seeded radial/spiral trajectories, Gaussian-bump coil maps, phantom k-space + 30 dB noise)",
         "config": {"workload": "64 uneven 2D CG-SENSE problems, one captured graph", "N": [128, 192, 256],
                    "coils": [8, 16, 32], "samples_per_coil": [20000, 130000], "cg_iters": 10, "width": 4,
-                   "oversamp": 1.25, "samples_per_item": str(samples_per_item)},
+                   "oversamp": 1.25, "samples_per_item": str(samples_per_item), "tile": B.tile,
+                   "tile_tuning_ms": B.tile_times},
         "plain_loop": {"value": 64 / t_plain, "unit": "recons/s", "ms": t_plain * 1e3,
                        "path": "per problem: sense_linop + CG_SENSE_recon (fused single-problem HIP operators)"},
         "speedup_vs_plain": t_plain / t_batch,

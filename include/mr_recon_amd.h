@@ -27,6 +27,14 @@ int mra_version(void);
  * one launch serve many uneven problems. */
 int mra_geom_size(void);
 int mra_default_tile(int ndim);
+/* Tile edge (grid points) of the work decomposition. The kernels are compiled for 2D tiles
+ * of 8, 16 (default) and 32 and for one edge in 1D (256) and 3D (8). mra_set_tile selects the
+ * edge that the calling thread's subsequent plan, Geom, subgrid-size, coil-chunk and kernel
+ * calls for ndim use (returns the previous edge, or -1 for an edge without kernels); a plan
+ * and every call on it must use the same edge. No counterpart in the reference (its Triton
+ * kernels have no tiles); it exists so that an autotuner can choose the tile. */
+int mra_tile(int ndim);
+int mra_set_tile(int ndim, int tile);
 long long mra_subgrid_points(int ndim, int P);
 int mra_max_coil_chunk(int ndim, int P);
 int mra_grad_samples_per_block(void);

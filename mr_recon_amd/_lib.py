@@ -37,6 +37,8 @@ SIGNATURES = {
     "mra_geom_set_dcf_plan_order": (_i, [_vp, _i]),
     "mra_geom_set_ksp_plan_order": (_i, [_vp, _i]),
     "mra_default_tile": (_i, [_i]),
+    "mra_tile": (_i, [_i]),
+    "mra_set_tile": (_i, [_i, _i]),
     "mra_grad_samples_per_block": (_i, []),
     "mra_subgrid_points": (_ll, [_i, _i]),
     "mra_max_coil_chunk": (_i, [_i, _i]),
@@ -143,6 +145,26 @@ def ptr(t: torch.Tensor | None):
 
 def stream_ptr(device: torch.device | None = None):
     return _vp(torch.cuda.current_stream(device).cuda_stream)
+
+
+class tile_scope:
+    """Select the kernels' tile edge for ndim-dimensional calls of this thread (the C
+    library keeps one per thread) and restore the previous one on exit."""
+
+    __slots__ = ("ndim", "tile", "prev")
+
+    def __init__(self, ndim: int, tile: int):
+        self.ndim, self.tile = int(ndim), int(tile)
+
+    def __enter__(self):
+        self.prev = lib().mra_set_tile(self.ndim, self.tile)
+        if self.prev < 0:
+            raise ValueError(lib().mra_last_error().decode())
+        return self
+
+    def __exit__(self, *exc):
+        lib().mra_set_tile(self.ndim, self.prev)
+        return False
 
 
 def ints(vals) -> C.Array:

@@ -1,13 +1,15 @@
 """Autotuner for the work decomposition of the KB interp / gridding kernels.
 
-The tile edge T is fixed per dimensionality at compile time (3D 8, 2D 16, 1D
-256); what varies with the trajectory is how many sorted samples a work item
-takes (``samples_per_item``): small items spread hot tiles (radial centres)
-over more blocks, large items cut the per-item subgrid loads and the gridding
-scratch. ``tune_samples_per_item`` times interp + gridding on the caller's own
-trajectory for each candidate and keeps the fastest; results are cached in a
-JSON table keyed by (ndim, kernel points, grid size, samples/tile bucket,
-batch bucket) so later plans of the same class skip the search.
+Two knobs vary with the trajectory. ``samples_per_item``: small items spread hot
+tiles (radial centres) over more blocks, large items cut the per-item subgrid
+loads and the gridding scratch. The tile edge (2D: 8, 16 or 32 grid points; the
+kernels are compiled for each): small tiles keep the LDS subgrids small and the
+blocks many, large ones cut the subgrid halo (a 16 x 16 tile's subgrid at P = 5 is
+1.56x its area, a 32 x 32 one's 1.27x) and with it the gridding scratch.
+``tune_work`` times interp + gridding on the caller's own trajectory for every
+candidate pair and keeps the fastest; results are cached in a JSON table keyed by
+(ndim, kernel points, grid size, samples/tile bucket, batch bucket, tile choice)
+so later plans of the same class skip the search.
 """
 
 from __future__ import annotations
@@ -21,10 +23,11 @@ from typing import Iterable, Optional
 
 import torch
 
-from . import ops
+from . import _lib, ops
 from .plan import SamplePlan, kernel_points
 
 CANDIDATES = (256, 512, 1024, 2048, 4096, 8192)
+TILES = {1: (256,), 2: (8, 16, 32), 3: (8,)}  # tile edges the kernels are compiled for
 _CACHE_PATH = Path(os.environ.get("MR_RECON_AMD_TUNE_CACHE",
                                   Path.home() / ".cache" / "mr_recon_amd" / "tune.json"))
 _lock = threading.Lock()
@@ -68,35 +71,53 @@ def _time(fn, reps: int = 3) -> float:
     return a.elapsed_time(b) / reps
 
 
-def tune_samples_per_item(coords: torch.Tensor, grid_size, width: float, beta: float, nbatch: int, *,
-                          scale=None, shift=None, wrap=None, candidates: Iterable[int] = CANDIDATES,
-                          use_cache: bool = True, reps: int = 3) -> int:
-    """Fastest ``samples_per_item`` for interp + gridding of ``nbatch`` grids on this trajectory."""
+def tune_work(coords: torch.Tensor, grid_size, width: float, beta: float, nbatch: int, *, scale=None, shift=None,
+              wrap=None, tiles: Optional[Iterable[int]] = None, samples: Optional[Iterable[int]] = None,
+              tune_tile: bool = True, use_cache: bool = True, reps: int = 3) -> tuple[int, int]:
+    """Fastest (tile edge, samples_per_item) for interp + gridding of ``nbatch`` grids on this
+    trajectory. ``tiles`` / ``samples`` restrict the candidates; without ``tune_tile`` the
+    tile is the default (or the one given)."""
     D = len(grid_size)
     M = int(math.prod(coords.shape[:-1]))
-    key = tune_key(D, width, grid_size, M, nbatch)
+    L = _lib.lib()
+    tiles = tuple(tiles) if tiles is not None else (TILES[D] if tune_tile else (int(L.mra_default_tile(D)),))
+    samples = tuple(samples) if samples is not None else CANDIDATES
+    key = tune_key(D, width, grid_size, M, nbatch) + "-t" + "_".join(map(str, tiles)) + \
+        "-s" + "_".join(map(str, samples))
     with _lock:
         table = _load()
         if use_cache and key in table:
-            return int(table[key]["samples_per_item"])
+            return int(table[key]["tile"]), int(table[key]["samples_per_item"])
     dev = coords.device
     grid = torch.randn((nbatch, *grid_size), dtype=torch.complex64, device=dev)
     ksp = torch.randn((nbatch, M), dtype=torch.complex64, device=dev)
     out = torch.empty_like(grid)
     times = {}
-    for S in candidates:
-        plan = SamplePlan(coords, grid_size, width, beta, scale=scale, shift=shift, wrap=wrap, samples_per_item=S)
-        scratch = torch.empty(ops.gridding_scratch_elems(plan, nbatch), dtype=torch.complex64, device=dev)
-        t_i = _time(lambda: ops.interp(grid, plan, out=ksp), reps)
-        t_g = _time(lambda: ops.gridding(ksp, plan, out, scratch=scratch), reps)
-        times[S] = t_i + t_g
-        del scratch, plan
+    for T in tiles:
+        for S in samples:
+            plan = SamplePlan(coords, grid_size, width, beta, scale=scale, shift=shift, wrap=wrap, tile=T,
+                              samples_per_item=S)
+            scratch = torch.empty(ops.gridding_scratch_elems(plan, nbatch), dtype=torch.complex64, device=dev)
+            t_i = _time(lambda: ops.interp(grid, plan, out=ksp), reps)
+            t_g = _time(lambda: ops.gridding(ksp, plan, out, scratch=scratch), reps)
+            times[(T, S)] = t_i + t_g
+            del scratch, plan
     best = min(times, key=times.get)
     with _lock:
         table = _load()
-        table[key] = {"samples_per_item": int(best), "ms": {str(k): round(v, 4) for k, v in times.items()}}
+        table[key] = {"tile": int(best[0]), "samples_per_item": int(best[1]),
+                      "ms": {f"{t}/{s}": round(v, 4) for (t, s), v in times.items()}}
         _save()
-    return int(best)
+    return int(best[0]), int(best[1])
+
+
+def tune_samples_per_item(coords: torch.Tensor, grid_size, width: float, beta: float, nbatch: int, *,
+                          scale=None, shift=None, wrap=None, candidates: Iterable[int] = CANDIDATES,
+                          use_cache: bool = True, reps: int = 3) -> int:
+    """Fastest ``samples_per_item`` for interp + gridding of ``nbatch`` grids on this
+    trajectory, at the default tile edge."""
+    return tune_work(coords, grid_size, width, beta, nbatch, scale=scale, shift=shift, wrap=wrap,
+                     samples=candidates, tune_tile=False, use_cache=use_cache, reps=reps)[1]
 
 
 def tuned_table() -> dict:

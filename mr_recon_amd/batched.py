@@ -50,7 +50,7 @@ class _Group:
     the batch-wide buffers (grid, k-space, samples, items, scratch) are set by BatchedSense."""
 
     def __init__(self, idx: list[int], problems: Sequence[Problem], N: int, oversamp: float, width: float,
-                 samples_per_item: Optional[int]):
+                 samples_per_item: Optional[int], tile: Optional[int] = None):
         self.idx = idx
         self.N = N
         self.im = (N, N)
@@ -58,7 +58,7 @@ class _Group:
         self.device = dev
         cmax = max(int(problems[i].mps.shape[0]) for i in idx)
         self.nufft = hip_nufft(self.im, oversamp=oversamp, width=width, samples_per_item=samples_per_item,
-                               tune_batch=cmax)
+                               tune_batch=cmax, tile=tile)
         self.G = self.nufft.os_shape()
         self.npts_grid = math.prod(self.G)
         self.apod = self.nufft.apod_table(dev)
@@ -107,7 +107,13 @@ class BatchedSense:
     tensor (nimg, N, N) per group."""
 
     def __init__(self, problems: Sequence[Problem], oversamp: float = 1.25, width: float = 4,
-                 samples_per_item: Optional[int] = None):
+                 samples_per_item: Optional[int] = None, tile=None):
+        """``tile``: the 2D tile edge of every problem's plan (one launch serves them all):
+        None for the default, an int, or "auto" -- each compiled edge is tried on the
+        batch's own normal operator and the fastest kept."""
+        self.tile_times = None  # ms per normal-operator application for each tried edge
+        if tile == "auto":
+            tile, self.tile_times = self._tune_tile(problems, oversamp, width, samples_per_item)
         self.problems = list(problems)
         sizes = {}
         for i, pr in enumerate(self.problems):
@@ -115,15 +121,37 @@ class BatchedSense:
             if pr.mps.ndim != 3 or pr.mps.shape[1] != pr.mps.shape[2]:
                 raise ValueError("batched problems are 2D with square images: mps (C, N, N)")
             sizes.setdefault(int(pr.mps.shape[1]), []).append(i)
-        self.groups = [_Group(idx, self.problems, N, oversamp, width, samples_per_item)
+        self.groups = [_Group(idx, self.problems, N, oversamp, width, samples_per_item, tile)
                        for N, idx in sorted(sizes.items())]
         g0 = self.groups[0]
         if any(g.P != g0.P or g.tile != g0.tile for g in self.groups):
             raise ValueError("batched problems must share the kernel width and tile")
         self.P = g0.P
+        self.tile = g0.tile
         self.device = g0.device
         self._assemble()
         self._graph = None
+
+    @staticmethod
+    def _tune_tile(problems, oversamp, width, samples_per_item, reps: int = 3):
+        from .autotune import TILES
+
+        times = {}
+        for T in TILES[2]:
+            B = BatchedSense(problems, oversamp, width, samples_per_item, tile=T)
+            xs = [torch.randn(pr.mps.shape[1:], dtype=complex_dtype, device=pr.mps.device) for pr in problems]
+            Xs = B.stack_images(xs)
+            B.normal(Xs)
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(reps):
+                B.normal(Xs)
+            b.record()
+            torch.cuda.synchronize()
+            times[T] = a.elapsed_time(b) / reps
+            del B, Xs
+        return min(times, key=times.get), times
 
     def _assemble(self):
         """One Geom table, item list, sample arrays, scratch and grid buffer for the batch."""
@@ -181,7 +209,8 @@ class BatchedSense:
             full = [(d if d is not None else torch.ones(m, device=dev)).reshape(-1).float() for d, m in zip(dcfs, ms)]
             self.dcf = torch.cat(full).contiguous()
             self.dcf_po = torch.cat([d[pm.long()] for d, pm in zip(full, perm)]).contiguous()
-        self.cc = max(1, min(maxC, L.mra_max_coil_chunk(2, self.P)))
+        with _lib.tile_scope(2, self.tile):
+            self.cc = max(1, min(maxC, L.mra_max_coil_chunk(2, self.P)))
         self.nchunks = -(-maxC // self.cc)
         self.scratch = torch.empty(max(1, scratch), dtype=complex_dtype, device=dev)
         self.gridbuf = torch.empty(max(1, grid_base), dtype=complex_dtype, device=dev)
@@ -202,8 +231,10 @@ class BatchedSense:
                 view.copy_(_fftn(view, 2))
         Y = torch.empty(self.ksp_len, dtype=complex_dtype, device=self.device)
         geoms = self.geoms_po if plan_order else self.geoms
-        check(lib().mra_interp(2, self.P, ptr(self.gridbuf), ptr(Y), ptr(self.wsort), ptr(self.lbase),
-                               ptr(self.perm), ptr(geoms), ptr(self.items), self.nitems, self.cc, self.nchunks, st))
+        with _lib.tile_scope(2, self.tile):
+            check(lib().mra_interp(2, self.P, ptr(self.gridbuf), ptr(Y), ptr(self.wsort), ptr(self.lbase),
+                                   ptr(self.perm), ptr(geoms), ptr(self.items), self.nitems, self.cc, self.nchunks,
+                                   st))
         return Y
 
     def adjoint(self, Y: torch.Tensor, scale: float = 1.0, plan_order: bool = False) -> list[torch.Tensor]:
@@ -211,12 +242,13 @@ class BatchedSense:
         if Y.numel() != self.ksp_len or Y.dtype != complex_dtype or not Y.is_contiguous():
             raise ValueError(f"adjoint: k-space must be contiguous complex64 with {self.ksp_len} elements")
         geoms, dcf = (self.geoms_po, self.dcf_po) if plan_order else (self.geoms, self.dcf)
-        check(lib().mra_grid_partial(2, self.P, ptr(Y), ptr(dcf), ptr(self.wsort), ptr(self.lbase),
-                                     ptr(self.perm), ptr(geoms), ptr(self.items), self.nitems, self.maxC,
-                                     ptr(self.scratch), st))
-        for g in self.groups:
-            check(lib().mra_grid_reduce(2, self.P, ptr(self.gridbuf), ptr(self.scratch), ptr(self.geoms), g.prob0,
-                                        g.nimg, ptr(self.tile_items), g.ntiles, g.maxC, 0, st))
+        with _lib.tile_scope(2, self.tile):
+            check(lib().mra_grid_partial(2, self.P, ptr(Y), ptr(dcf), ptr(self.wsort), ptr(self.lbase),
+                                         ptr(self.perm), ptr(geoms), ptr(self.items), self.nitems, self.maxC,
+                                         ptr(self.scratch), st))
+            for g in self.groups:
+                check(lib().mra_grid_reduce(2, self.P, ptr(self.gridbuf), ptr(self.scratch), ptr(self.geoms),
+                                            g.prob0, g.nimg, ptr(self.tile_items), g.ntiles, g.maxC, 0, st))
         Xs = []
         for g in self.groups:
             view = g.grid_view(self.gridbuf)

@@ -44,11 +44,18 @@ struct TileShape {
   static constexpr int TN = ipow(T, D);             // tile points
 };
 
-// Tile edge per dimensionality (grid points).
+// Tile edge per dimensionality (grid points): the default, and the edges the kernels are
+// compiled for (2D: 8, 16 or 32, chosen per plan; 1D and 3D: one edge).
 template <int D> struct TileCfg;
 template <> struct TileCfg<1> { static constexpr int T = 256; };
 template <> struct TileCfg<2> { static constexpr int T = 16; };
 template <> struct TileCfg<3> { static constexpr int T = 8; };
+constexpr bool tile_compiled(int D, int T) {
+  return D == 1 ? T == 256 : D == 2 ? (T == 8 || T == 16 || T == 32) : D == 3 ? T == 8 : false;
+}
+// Tile edge the calls of this thread use, per dimensionality (set by mra_set_tile). Plans,
+// Geom records and kernel calls on one trajectory must use the same edge.
+static thread_local int g_tile[4] = {0, TileCfg<1>::T, TileCfg<2>::T, TileCfg<3>::T};
 
 #ifndef MRA_INTERP_LDS_KB
 #define MRA_INTERP_LDS_KB 32
@@ -1370,9 +1377,8 @@ static int g_grid_kernel = 1;
 #endif
 static int g_interp_pairs = MRA_INTERP_PAIRS;
 
-template <int D, int P>
+template <int D, int P, int T>
 static int launch_op(const LaunchArgs& a) {
-  constexpr int T = TileCfg<D>::T;
   using TS = TileShape<D, P, T>;
   switch (a.op) {
     case Op::Weights:
@@ -1449,17 +1455,17 @@ static int launch_op(const LaunchArgs& a) {
   return 0;
 }
 
-template <int D>
+template <int D, int T>
 static int dispatch_p(int P, const LaunchArgs& a) {
   switch (P) {
-    case 2: return launch_op<D, 2>(a);
-    case 3: return launch_op<D, 3>(a);
-    case 4: return launch_op<D, 4>(a);
-    case 5: return launch_op<D, 5>(a);
-    case 6: return launch_op<D, 6>(a);
-    case 7: return launch_op<D, 7>(a);
-    case 8: return launch_op<D, 8>(a);
-    case 9: return launch_op<D, 9>(a);
+    case 2: return launch_op<D, 2, T>(a);
+    case 3: return launch_op<D, 3, T>(a);
+    case 4: return launch_op<D, 4, T>(a);
+    case 5: return launch_op<D, 5, T>(a);
+    case 6: return launch_op<D, 6, T>(a);
+    case 7: return launch_op<D, 7, T>(a);
+    case 8: return launch_op<D, 8, T>(a);
+    case 9: return launch_op<D, 9, T>(a);
     default:
       set_last_error("kernel width unsupported: floor(width)+1 must be in 2..9");
       return 1;
@@ -1468,9 +1474,14 @@ static int dispatch_p(int P, const LaunchArgs& a) {
 
 static int dispatch(int ndim, int P, const LaunchArgs& a) {
   switch (ndim) {
-    case 1: return dispatch_p<1>(P, a);
-    case 2: return dispatch_p<2>(P, a);
-    case 3: return dispatch_p<3>(P, a);
+    case 1: return dispatch_p<1, TileCfg<1>::T>(P, a);
+    case 2:
+      switch (g_tile[2]) {
+        case 8: return dispatch_p<2, 8>(P, a);
+        case 32: return dispatch_p<2, 32>(P, a);
+        default: return dispatch_p<2, 16>(P, a);
+      }
+    case 3: return dispatch_p<3, TileCfg<3>::T>(P, a);
     default:
       set_last_error("ndim must be 1, 2 or 3");
       return 1;
@@ -1478,7 +1489,7 @@ static int dispatch(int ndim, int P, const LaunchArgs& a) {
 }
 
 static int subgrid_points(int ndim, int P) {
-  const int T = ndim == 1 ? TileCfg<1>::T : ndim == 2 ? TileCfg<2>::T : TileCfg<3>::T;
+  const int T = ndim >= 1 && ndim <= 3 ? g_tile[ndim] : 1;
   return ipow(T + P - 1, ndim);
 }
 
@@ -1611,6 +1622,18 @@ int mra_default_tile(int ndim) {
   }
 }
 
+int mra_tile(int ndim) { return ndim >= 1 && ndim <= 3 ? g_tile[ndim] : 0; }
+
+int mra_set_tile(int ndim, int tile) {
+  if (ndim < 1 || ndim > 3 || !tile_compiled(ndim, tile)) {
+    set_last_error("set_tile: no kernels compiled for this tile edge (2D: 8, 16, 32; 1D: 256; 3D: 8)");
+    return -1;
+  }
+  const int old = g_tile[ndim];
+  g_tile[ndim] = tile;
+  return old;
+}
+
 int mra_grad_samples_per_block() { return kThreads * kGradK; }
 
 // Subgrid points per tile (scratch slot size of gridding) for (ndim, P).
@@ -1633,7 +1656,7 @@ int mra_geom_fill(void* dst, long long grid_off, long long grid_stride, long lon
                   int tile_items_off) {
   MRA_TRY_BEGIN
   if (ndim < 1 || ndim > kMaxDim) MRA_FAIL("geom: ndim must be 1..3");
-  if (tile != mra_default_tile(ndim)) MRA_FAIL("geom: tile size does not match the compiled kernels");
+  if (tile != mra_tile(ndim)) MRA_FAIL("geom: tile edge differs from the one selected for this thread (mra_set_tile)");
   Geom g{};
   g.grid_off = grid_off;
   g.grid_stride = grid_stride;

@@ -174,10 +174,12 @@ class hip_nufft(NUFFT):
 
     def __init__(self, im_size: tuple, oversamp: Optional[float] = 1.25, width: Optional[int] = 6,
                  beta: Optional[float] = None, apodize: Optional[bool] = True,
-                 samples_per_item=None, tune_batch: Optional[int] = None):
+                 samples_per_item=None, tune_batch: Optional[int] = None, tile=None):
         """``samples_per_item``: work-item size of the KB kernels (None: a size heuristic;
         "auto": timed by the autotuner on the first plan of each trajectory class, for
-        ``tune_batch`` grids per call)."""
+        ``tune_batch`` grids per call). ``tile``: tile edge of the plans (None: the default
+        for the dimensionality; an int; "auto": chosen by the autotuner together with the
+        work-item size, 2D only -- 1D and 3D have one compiled edge)."""
         super().__init__(im_size)
         self.oversamp = oversamp
         self.width = width
@@ -185,6 +187,7 @@ class hip_nufft(NUFFT):
         self.beta = kb_beta(width, oversamp) if beta is None else beta
         self.samples_per_item = samples_per_item
         self.tune_batch = tune_batch
+        self.tile = tile
         self._plans = _PlanCache()
         self._apod_cache: dict = {}
 
@@ -244,15 +247,18 @@ class hip_nufft(NUFFT):
             t = t.to(torch.float32)
 
         def build():
-            S = self.samples_per_item
-            if S == "auto":
-                from .autotune import tune_samples_per_item
+            S, T = self.samples_per_item, self.tile
+            if S == "auto" or T == "auto":
+                from .autotune import tune_work
 
-                S = tune_samples_per_item(t.reshape(-1, len(G)), G, self.width, self.beta, self.tune_batch or 1,
-                                          scale=sc, shift=sh, wrap=wrap)
-            return SamplePlan(t, G, self.width, self.beta, scale=sc, shift=sh, wrap=wrap, samples_per_item=S)
+                T, S = tune_work(t.reshape(-1, len(G)), G, self.width, self.beta, self.tune_batch or 1,
+                                 scale=sc, shift=sh, wrap=wrap, tiles=None if T in (None, "auto") else (T,),
+                                 samples=None if S in (None, "auto") else (S,), tune_tile=T == "auto")
+            return SamplePlan(t, G, self.width, self.beta, scale=sc, shift=sh, wrap=wrap, tile=T,
+                              samples_per_item=S)
 
-        return self._plans.get(t, (G, wrap, im_size, self.width, self.beta, self.samples_per_item), build)
+        return self._plans.get(t, (G, wrap, im_size, self.width, self.beta, self.samples_per_item, self.tile),
+                               build)
 
     # -- split methods (reference layout) -------------------------------
     # -- 4D / 5D: the reference's composition with torch FFTs and the direct KB kernels -------

@@ -41,8 +41,9 @@ def interp(grid: torch.Tensor, plan: SamplePlan, out_scale: float = 1.0, out: to
     g = geom if geom is not None else plan.geom(B, plan.npts, out_scale)
     cc = plan.coil_chunk(B)
     nch = -(-B // cc)
-    check(lib().mra_interp(D, plan.P, ptr(grid), ptr(out), ptr(plan.wsort), ptr(plan.lbase), ptr(plan.perm),
-                           ptr(g), ptr(plan.items), plan.nitems, cc, nch, stream_ptr(grid.device)))
+    with _lib.tile_scope(plan.ndim, plan.tile):
+        check(lib().mra_interp(D, plan.P, ptr(grid), ptr(out), ptr(plan.wsort), ptr(plan.lbase), ptr(plan.perm),
+                               ptr(g), ptr(plan.items), plan.nitems, cc, nch, stream_ptr(grid.device)))
     return out
 
 
@@ -95,9 +96,10 @@ def gridding_partial(ksp: torch.Tensor, plan: SamplePlan, nbatch: int, dcf: torc
         scratch = torch.empty(need, dtype=torch.complex64, device=ksp.device)
     elif scratch.numel() < need or scratch.dtype != torch.complex64 or not scratch.is_contiguous():
         raise ValueError(f"gridding scratch must be contiguous complex64 with >= {need} elements")
-    check(lib().mra_grid_partial(plan.ndim, plan.P, ptr(ksp), ptr(dcf), ptr(plan.wsort), ptr(plan.lbase),
-                                 ptr(plan.perm), ptr(g), ptr(plan.items), plan.nitems, B, ptr(scratch),
-                                 stream_ptr(ksp.device)))
+    with _lib.tile_scope(plan.ndim, plan.tile):
+        check(lib().mra_grid_partial(plan.ndim, plan.P, ptr(ksp), ptr(dcf), ptr(plan.wsort), ptr(plan.lbase),
+                                     ptr(plan.perm), ptr(g), ptr(plan.items), plan.nitems, B, ptr(scratch),
+                                     stream_ptr(ksp.device)))
     return scratch
 
 
@@ -109,8 +111,9 @@ def gridding_reduce(scratch: torch.Tensor, plan: SamplePlan, out: torch.Tensor,
     if scratch.numel() < gridding_scratch_elems(plan, B):
         raise ValueError("gridding scratch too small for this batch")
     g = plan.geom(B, plan.npts, 1.0)
-    check(lib().mra_grid_reduce(plan.ndim, plan.P, ptr(out), ptr(scratch), ptr(g), 0, 1, ptr(plan.tile_items),
-                                plan.ntiles, B, int(accumulate), stream_ptr(out.device)))
+    with _lib.tile_scope(plan.ndim, plan.tile):
+        check(lib().mra_grid_reduce(plan.ndim, plan.P, ptr(out), ptr(scratch), ptr(g), 0, 1, ptr(plan.tile_items),
+                                    plan.ntiles, B, int(accumulate), stream_ptr(out.device)))
     return out
 
 
@@ -132,10 +135,11 @@ def gridding_noalloc(ksp: torch.Tensor, plan: SamplePlan, out: torch.Tensor, dcf
             raise ValueError("dcf must be float32 with one entry per sample")
         dcf = dcf.contiguous()
     gh = plan.geom_host(B, out_scale)
-    check(lib().mra_grid_noalloc(plan.ndim, plan.P, ptr(ksp), ptr(dcf), ptr(plan.wsort), ptr(plan.lbase),
-                                 ptr(plan.perm), ctypes.addressof(gh), ptr(plan.items), plan.nitems, B,
-                                 ptr(plan.tile_items), plan.ntiles, ptr(out), int(accumulate),
-                                 stream_ptr(out.device)))
+    with _lib.tile_scope(plan.ndim, plan.tile):
+        check(lib().mra_grid_noalloc(plan.ndim, plan.P, ptr(ksp), ptr(dcf), ptr(plan.wsort), ptr(plan.lbase),
+                                     ptr(plan.perm), ctypes.addressof(gh), ptr(plan.items), plan.nitems, B,
+                                     ptr(plan.tile_items), plan.ntiles, ptr(out), int(accumulate),
+                                     stream_ptr(out.device)))
     return out
 
 
@@ -164,8 +168,9 @@ def interp_grad(grid: torch.Tensor, a: torch.Tensor, plan: SamplePlan, out_scale
         raise ValueError("interp_grad: a must hold B*M samples")
     gt = torch.empty((plan.npts, D), dtype=torch.float32, device=grid.device)
     g = plan.geom(B, plan.npts, out_scale, ksp_plan_order=a_plan_order)
-    check(lib().mra_interp_grad(D, plan.P, ptr(grid), ptr(a), ptr(gt), ptr(plan.tsort), ptr(plan.perm), ptr(g),
-                                ptr(plan.items), plan.nitems, plan.S, stream_ptr(grid.device)))
+    with _lib.tile_scope(plan.ndim, plan.tile):
+        check(lib().mra_interp_grad(D, plan.P, ptr(grid), ptr(a), ptr(gt), ptr(plan.tsort), ptr(plan.perm), ptr(g),
+                                    ptr(plan.items), plan.nitems, plan.S, stream_ptr(grid.device)))
     return gt
 
 

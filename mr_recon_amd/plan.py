@@ -64,7 +64,8 @@ class SamplePlan:
         units (the plain ``ungrid``/``grid`` surface)
     wrap : per-dim integer added to a grid index before it wraps mod G. The
         NUFFT uses ``-(G//2)`` so kernels address the unshifted FFT output.
-    tile : tile edge in grid points (default per dimensionality)
+    tile : tile edge in grid points (default: the edge selected for this thread, see
+        ``_lib.tile_scope``; kernels are compiled for 2D edges 8, 16, 32, 1D 256, 3D 8)
     samples_per_item : max samples per work item (default: about 1k items)
     """
 
@@ -97,7 +98,9 @@ class SamplePlan:
         self.npts = int(math.prod(self.coords_shape))
         if self.npts <= 0:
             raise ValueError("trajectory has no samples")
-        self.tile = int(tile or L.mra_default_tile(D))
+        self.tile = int(tile or L.mra_tile(D))
+        with _lib.tile_scope(D, self.tile):  # raises for an edge without kernels
+            pass
         self.ntile = [(g + self.tile - 1) // self.tile for g in self.grid_size]
         self.ntiles = int(math.prod(self.ntile))
         self.S = int(samples_per_item or default_samples_per_item(self.npts))
@@ -134,14 +137,15 @@ class SamplePlan:
         # KB weights (D x P per sorted sample) and subgrid offsets, once per trajectory
         self.wsort = torch.empty(self.npts * D * self.P, dtype=torch.float32, device=dev)
         self.lbase = torch.empty(self.npts, dtype=torch.int32, device=dev)
-        check(L.mra_plan_weights_dims(D, self.P, ptr(self.tsort), self.npts, _lib.ints(self.grid_size),
-                                      _lib.ints(self.wrap), self.width, _lib.floats(self.betas), ptr(self.wsort),
-                                      ptr(self.lbase), st))
+        with _lib.tile_scope(D, self.tile):
+            check(L.mra_plan_weights_dims(D, self.P, ptr(self.tsort), self.npts, _lib.ints(self.grid_size),
+                                          _lib.ints(self.wrap), self.width, _lib.floats(self.betas),
+                                          ptr(self.wsort), ptr(self.lbase), st))
+            self.subgrid = int(L.mra_subgrid_points(D, self.P))
+            self.max_coil_chunk = int(L.mra_max_coil_chunk(D, self.P))
         # The only host synchronisation of a plan: the number of work items, which
         # sizes the launches and the gridding scratch.
         self.nitems = int(item_off[nkeys].item())
-        self.subgrid = int(L.mra_subgrid_points(D, self.P))
-        self.max_coil_chunk = int(L.mra_max_coil_chunk(D, self.P))
         # The workspaces go back to the caching allocator here; PyTorch only
         # hands them out again to work ordered after these kernels on this stream.
         del work, nitems_ws, scan_ws2, flat
@@ -188,10 +192,12 @@ def pack_geoms(entries: list[dict]):
     buf = (C.c_ubyte * (gsz * len(entries)))()
     for i, e in enumerate(entries):
         D = len(e["grid_size"])
-        check(L.mra_geom_fill(C.addressof(buf) + i * gsz, e["grid_off"], e["grid_stride"], e["ksp_off"],
-                              e["ksp_stride"], e["samp_off"], e["nbatch"], e["npts"], D, _lib.ints(e["grid_size"]),
-                              e["tile"], _lib.ints(e["wrap"]), e["beta"], e["width"], e["out_scale"],
-                              e.get("item_base", 0), e.get("scratch_off", 0), e.get("tile_items_off", 0)))
+        with _lib.tile_scope(D, e["tile"]):
+            check(L.mra_geom_fill(C.addressof(buf) + i * gsz, e["grid_off"], e["grid_stride"], e["ksp_off"],
+                                  e["ksp_stride"], e["samp_off"], e["nbatch"], e["npts"], D,
+                                  _lib.ints(e["grid_size"]), e["tile"], _lib.ints(e["wrap"]), e["beta"], e["width"],
+                                  e["out_scale"], e.get("item_base", 0), e.get("scratch_off", 0),
+                                  e.get("tile_items_off", 0)))
         if e.get("dcf_plan_order"):
             check(L.mra_geom_set_dcf_plan_order(C.addressof(buf) + i * gsz, 1))
         if e.get("ksp_plan_order"):

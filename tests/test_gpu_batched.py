@@ -53,6 +53,36 @@ def test_batched_operators_match_plain(dev):
         assert rel_l2(Z[i], A.adjoint(Ys[sl].reshape(ksp[i].shape))) < 1e-5
 
 
+@pytest.mark.parametrize("tile", [8, 32])
+def test_batched_tile_edges_match_plain(dev, tile):
+    """A batch planned with 2D tile edge 8 or 32 (one edge for every problem, one launch
+    serves them all) against plain operators with the same edge."""
+    from mr_recon_amd.batched import BatchedSense
+    from mr_recon_amd.fourier import hip_nufft
+    from mr_recon_amd.linops import sense_linop
+
+    probs, ksp = _problems(dev, seed=7)
+    B = BatchedSense(probs, oversamp=1.25, width=4, tile=tile)
+    assert B.tile == tile and all(p.tile == tile for g in B.groups for p in g.plans)
+    xs = [synth.random_image(tuple(pr.mps.shape[1:]), seed=40 + i).to(dev) for i, pr in enumerate(probs)]
+    Y = B.forward(B.stack_images(xs))
+    Z = B.unstack_images(B.adjoint(B.stack_ksp(ksp)))
+    for i, pr in enumerate(probs):
+        N = pr.mps.shape[1]
+        A = sense_linop(pr.trj, pr.mps, dcf=pr.dcf, nufft=hip_nufft((N, N), oversamp=1.25, width=4, tile=tile))
+        assert rel_l2(Y[B.ksp_slice(i)], A.forward(xs[i]).reshape(-1)) < 1e-5
+        assert rel_l2(Z[i], A.adjoint(ksp[i])) < 1e-5
+
+
+def test_batched_tile_autotune(dev):
+    from mr_recon_amd import autotune
+    from mr_recon_amd.batched import BatchedSense
+
+    probs, _ = _problems(dev, seed=8)
+    B = BatchedSense(probs, oversamp=1.25, width=4, tile="auto")
+    assert B.tile in autotune.TILES[2]
+
+
 def test_batched_normal_keeps_ksp_in_plan_order(dev):
     """The batched normal operator keeps k-space in plan order: equal to adjoint(forward)."""
     from mr_recon_amd.batched import BatchedSense

@@ -312,3 +312,37 @@ def test_dense_hot_tile_split(dev):
     y = torch.randn((3, 20000), dtype=torch.complex64, generator=g)
     gr = grid(y.to(dev), k.to(dev), (64, 64), 6.0, kernel_params={"beta": 9.0})
     assert rel_l2(gr, interp_ref.grid(y, k, (64, 64), 6.0, beta=9.0)) < TOL
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+@pytest.mark.parametrize("G,width", [(72, 4.0), (50, 6.0), (37, 3.0)])
+def test_2d_tile_edges(dev, tile, G, width):
+    """The 2D kernels compiled for tile edges 8, 16 and 32: plans bit-exact against the
+    oracle's binning with that edge (grid sizes that are and are not multiples of it), and
+    gather / gridding against the oracle, a hot tile included."""
+    from mr_recon_amd.hip_interp import grid, ungrid
+    from mr_recon_amd.plan import SamplePlan
+
+    k = _case(2, G, width, 2500, seed=G + tile)
+    rng = np.random.default_rng(tile)
+    hot = torch.from_numpy((G / 2 + rng.random((1500, 2)) * 3).astype(np.float32))
+    k = torch.cat([k, hot])
+    plan = SamplePlan(k.to(dev), (G, G), width, 6.0, tile=tile, samples_per_item=700)
+    ts, perm, start, _ = obins.bin_plan(k.numpy(), (G, G), width, tile)
+    assert plan.tile == tile and plan.subgrid == (tile + plan.P - 1) ** 2
+    assert np.array_equal(plan.perm.cpu().numpy(), perm)
+    assert np.array_equal(plan.tsort.cpu().numpy().view(np.int32), ts.view(np.int32))
+    assert np.array_equal(plan.tile_start.cpu().numpy(), start)
+    from mr_recon_amd import _lib, ops
+
+    g = torch.Generator().manual_seed(tile)
+    vals = torch.randn((3, G, G), dtype=torch.complex64, generator=g)
+    samp = torch.randn((3, k.shape[0]), dtype=torch.complex64, generator=g)
+    out = ops.interp(vals.to(dev), plan)
+    assert rel_l2(out, interp_ref.ungrid(vals, k, width, beta=6.0)) < TOL
+    gout = torch.empty((3, G, G), dtype=torch.complex64, device=dev)
+    ops.gridding(samp.to(dev), plan, gout)
+    assert rel_l2(gout, interp_ref.grid(samp, k, (G, G), width, beta=6.0)) < TOL
+    with _lib.tile_scope(2, tile):  # the plain surface plans with the thread's edge
+        assert rel_l2(ungrid(vals.to(dev), k.to(dev), width, kernel_params={"beta": 6.0}), out) < 1e-6
+        assert rel_l2(grid(samp.to(dev), k.to(dev), (G, G), width, kernel_params={"beta": 6.0}), gout) < 1e-6

@@ -114,3 +114,25 @@ def test_toeplitz_approximates_normal(dev):
     a = hn.normal_toeplitz(x[:, None].to(dev), K)[:, 0]
     b = hn.adjoint(hn.forward(x.to(dev), trj.to(dev)), trj.to(dev))
     assert rel_l2(a, b) < 2e-3
+
+
+@pytest.mark.parametrize("tile", [8, 32, "auto"])
+def test_hip_nufft_tile_choice(dev, tile):
+    """hip_nufft with a 2D tile edge of 8 or 32, or tile and work-item size both chosen by
+    the autotuner: forward and adjoint against the oracle, and the chosen edge recorded."""
+    from mr_recon_amd import autotune
+    from mr_recon_amd.fourier import hip_nufft
+
+    im = (64, 64)
+    trj = synth.radial_2d(40, 128, 64)
+    x = synth.random_image((1, 3, *im), seed=4)
+    nf = hip_nufft(im, oversamp=2.0, width=6, tile=tile, samples_per_item="auto" if tile == "auto" else None)
+    ref = TritonNufftRef(im, oversamp=2.0, width=6)
+    y = nf.forward(x.to(dev), trj[None].to(dev))
+    assert rel_l2(y, ref.forward(x, trj[None])) < TOL
+    z = nf.adjoint(y, trj[None].to(dev))
+    assert rel_l2(z, ref.adjoint(ref.forward(x, trj[None]), trj[None])) < TOL
+    plan = nf.plan(trj.reshape(-1, 2).to(dev))
+    assert plan.tile in autotune.TILES[2]
+    if tile != "auto":
+        assert plan.tile == tile

@@ -46,6 +46,36 @@ def test_query_functions_without_gpu():
     assert L.mra_geom_size() % 8 == 0
 
 
+def test_tile_selection_is_per_thread_and_validated():
+    import threading
+
+    from mr_recon_amd import _lib
+
+    L = _lib.lib()
+    assert L.mra_tile(2) == 16
+    assert L.mra_set_tile(2, 12) == -1 and b"tile" in L.mra_last_error()  # no kernels for 12
+    assert L.mra_set_tile(3, 16) == -1 and L.mra_set_tile(1, 128) == -1
+    seen = {}
+
+    def other():
+        seen["before"] = L.mra_tile(2)
+        with _lib.tile_scope(2, 8):
+            seen["inside"] = L.mra_tile(2)
+            seen["subgrid"] = L.mra_subgrid_points(2, 5)
+
+    with _lib.tile_scope(2, 32):
+        assert L.mra_tile(2) == 32 and L.mra_subgrid_points(2, 5) == 36 ** 2
+        th = threading.Thread(target=other)
+        th.start()
+        th.join()
+        assert L.mra_tile(2) == 32  # the other thread's choice did not leak
+    assert L.mra_tile(2) == 16
+    assert seen == {"before": 16, "inside": 8, "subgrid": 12 ** 2}
+    with pytest.raises(ValueError):
+        with _lib.tile_scope(2, 24):
+            pass
+
+
 def test_error_path_reports_message():
     from mr_recon_amd import _lib
 
