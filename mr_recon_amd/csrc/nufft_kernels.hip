@@ -416,7 +416,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, kIn
 #define MRA_GRID_CPL 1
 #endif
 #ifndef MRA_GRID_NSMAX
-#define MRA_GRID_NSMAX 8  // most coil slots per wave
+#define MRA_GRID_NSMAX 12  // most coil slots per wave (12 over 8: batched 2D +14 %, same box)
 #endif
 
 // LDS padding (row pitch of the last dim += E in 3D, coil-slot stride += SP), from a
