@@ -25,7 +25,20 @@ import torch
 from . import _lib
 from ._lib import check, lib, ptr, stream_ptr
 
-CHUNK_LEN = 4096  # samples per binning chunk (one wave walks one chunk in order)
+MAX_DIGIT = 8192  # largest radix of one sort pass (mra_bin_build)
+
+
+def binning_chunk(npts: int, ntiles: int, tile: int, ndim: int) -> int:
+    """Samples per radix-sort chunk. One wave walks a chunk in order, so the chunks are the
+    parallelism of each scatter pass (4096-sample chunks left a 131 k-sample trajectory 32
+    waves); each pass's histogram holds digits x chunks counts. Aim for about 4096 chunks of
+    128..4096 samples, with at most 2^25 counts per histogram."""
+    digits = max(tile ** ndim, min(ntiles + 1, MAX_DIGIT))
+    c = -(-npts // 4096)
+    c = min(max(-(-c // 64) * 64, 128), 4096)
+    while c < 4096 and digits * -(-npts // c) > 1 << 25:
+        c *= 2
+    return min(c, 4096)
 
 
 def default_samples_per_item(npts: int) -> int:
@@ -110,7 +123,8 @@ class SamplePlan:
 
         flat = coords.reshape(-1, D).contiguous()
         dev = self.device
-        wwords = int(L.mra_bin_workspace(D, self.npts, self.ntiles, self.tile, CHUNK_LEN))
+        chunk = binning_chunk(self.npts, self.ntiles, self.tile, D)
+        wwords = int(L.mra_bin_workspace(D, self.npts, self.ntiles, self.tile, chunk))
         if wwords < 0:
             check(1)
         work = torch.empty(wwords, dtype=torch.int32, device=dev)
@@ -123,7 +137,7 @@ class SamplePlan:
         sh = _lib.floats(shift if shift is not None else [0.0] * D)
         st = stream_ptr(dev)
         check(L.mra_bin_build(D, ptr(flat), self.npts, _lib.ints(self.grid_size), sc, sh, int(apply),
-                              self.width, self.tile, _lib.ints(self.wrap), CHUNK_LEN, ptr(work), ptr(self.tsort),
+                              self.width, self.tile, _lib.ints(self.wrap), chunk, ptr(work), ptr(self.tsort),
                               ptr(self.perm), ptr(self.tile_start), st))
         self.max_items = nkeys + (self.npts + self.S - 1) // self.S
         self.items = torch.empty((self.max_items, 4), dtype=torch.int32, device=dev)
