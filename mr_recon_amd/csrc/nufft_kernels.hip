@@ -818,6 +818,10 @@ namespace mra {
 #define MRA_REDUCE_BATCH 4
 #endif
 constexpr int kReduceBatch = MRA_REDUCE_BATCH;
+#ifndef MRA_REDUCE_AHEAD
+#define MRA_REDUCE_AHEAD 4
+#endif
+constexpr int kReduceAhead = MRA_REDUCE_AHEAD;  // items whose loads are issued together
 
 // grid_reduce for 2D with every G >= SUB and even G[1]: the T x T tile has T*T/2 point
 // pairs (y, x), (y, x + 1), x even; thread t owns pair t % (T*T/2) for batch entries
@@ -867,9 +871,33 @@ __device__ __forceinline__ void reduce_pairs_2d(float2* __restrict__ grid, const
             acc[u].w += v.w;
           }
       }
+    // further items (tiles split into several: the dense k-space centre) kReduceAhead at a
+    // time, loads first, so that the hot tiles' blocks do not wait out one load per item;
+    // the sums keep item order
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-      for (int it = nxt[c]; it < end[c]; ++it) {
+    for (int c = 0; c < 4; ++c) {
+      int it = nxt[c];
+      for (; it + kReduceAhead <= end[c]; it += kReduceAhead) {
+        float4 v[kReduceAhead][2];
+#pragma unroll
+        for (int j = 0; j < kReduceAhead; ++j) {
+          const float2* slot = sbase + ((long long)(it + j - g.item_base) * g.nbatch + b0) * TS::SUBN + ex[c];
+#pragma unroll
+          for (int u = 0; u < 2; ++u)
+            v[j][u] = 2 * h + u < nb ? *(const float4*)(slot + (long long)(2 * h + u) * TS::SUBN)
+                                     : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int j = 0; j < kReduceAhead; ++j)
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            acc[u].x += v[j][u].x;
+            acc[u].y += v[j][u].y;
+            acc[u].z += v[j][u].z;
+            acc[u].w += v[j][u].w;
+          }
+      }
+      for (; it < end[c]; ++it) {
         const float2* slot = sbase + ((long long)(it - g.item_base) * g.nbatch + b0) * TS::SUBN + ex[c];
 #pragma unroll
         for (int u = 0; u < 2; ++u)
@@ -881,6 +909,7 @@ __device__ __forceinline__ void reduce_pairs_2d(float2* __restrict__ grid, const
             acc[u].w += v.w;
           }
       }
+    }
   } else
   for (int k0 = 0; k0 < ncand[0]; ++k0)
     for (int k1 = 0; k1 < ncand[1]; ++k1) {
@@ -951,7 +980,28 @@ __device__ __forceinline__ void reduce_pairs_3d(float2* __restrict__ grid, const
         const int e = (l0 * TS::SUB + l1) * TS::SUB + l2;
         const int stile = (c0 * g.ntile[1] + c1) * g.ntile[2] + c2;
         const int i1 = titems[stile + 1];
-        for (int it = titems[stile]; it < i1; ++it) {
+        int it = titems[stile];
+        if (ok)  // two items' loads at a time (split hot tiles); sums in item order
+          for (; it + 2 <= i1; it += 2) {
+            float4 v[2][RB];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+              const float2* slot = sbase + ((long long)(it + j - g.item_base) * g.nbatch + b0) * TS::SUBN + e;
+#pragma unroll
+              for (int b = 0; b < RB; ++b)
+                v[j][b] = b < nb ? *(const float4*)(slot + (long long)b * TS::SUBN) : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+              for (int b = 0; b < RB; ++b) {
+                acc[b].x += v[j][b].x;
+                acc[b].y += v[j][b].y;
+                acc[b].z += v[j][b].z;
+                acc[b].w += v[j][b].w;
+              }
+          }
+        for (; it < i1; ++it) {
           const float2* slot = sbase + ((long long)(it - g.item_base) * g.nbatch + b0) * TS::SUBN + e;
           if (ok) {
 #pragma unroll
