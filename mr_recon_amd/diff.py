@@ -56,8 +56,7 @@ class _Ctx:
         self.s_fwd = 1.0 / math.sqrt(math.prod(self.im_size))
         self.s_interp = 1.0 / nufft.width ** self.d
         self.s_adj = self.s_interp * self.s_fwd
-        sc, _ = nufft._scale_shift(self.im_size)
-        self.dt_dk = torch.tensor(sc, dtype=torch.float32, device=trj.device)
+        self.dt_dk = nufft.scale_table(trj.device, self.im_size)
 
     def grids(self, x: torch.Tensor) -> torch.Tensor:
         """FFT(pad(apod * S_c x)) / sqrt(prod N): (C, *G) in the unshifted layout."""

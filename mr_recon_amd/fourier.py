@@ -190,6 +190,7 @@ class hip_nufft(NUFFT):
         self.tile = tile
         self._plans = _PlanCache()
         self._apod_cache: dict = {}
+        self._scale_cache: dict = {}
 
     # -- geometry ------------------------------------------------------
     def os_shape(self, im_size=None):
@@ -221,6 +222,18 @@ class hip_nufft(NUFFT):
                 parts.append(self._apodization_func(crds / ceil(self.oversamp * n)))
             t = torch.cat(parts).to(torch.float32).to(device)
             self._apod_cache[key] = t
+        return t
+
+    def scale_table(self, device, im_size=None) -> torch.Tensor:
+        """Per-dim trajectory scale fl32(ceil(sN)/N) (dt/dk of the plan coordinates) as a
+        float32 device tensor, cached per (device, image size): building it is a host-to-device
+        copy, which waits for the stream."""
+        im_size = tuple(self.im_size if im_size is None else im_size)
+        key = (str(device), im_size)
+        t = self._scale_cache.get(key)
+        if t is None:
+            t = torch.tensor(self._scale_shift(im_size)[0], dtype=torch.float32).to(device)
+            self._scale_cache[key] = t
         return t
 
     def apodize_img(self, img: torch.Tensor) -> torch.Tensor:
