@@ -821,7 +821,11 @@ constexpr int kReduceBatch = MRA_REDUCE_BATCH;
 #ifndef MRA_REDUCE_AHEAD
 #define MRA_REDUCE_AHEAD 4
 #endif
-constexpr int kReduceAhead = MRA_REDUCE_AHEAD;  // items whose loads are issued together
+constexpr int kReduceAhead = MRA_REDUCE_AHEAD;  // items whose loads are issued together (2D)
+#ifndef MRA_REDUCE_AHEAD3
+#define MRA_REDUCE_AHEAD3 2
+#endif
+constexpr int kReduceAhead3 = MRA_REDUCE_AHEAD3;  // further items of one tile issued together (3D)
 
 // grid_reduce for 2D with every G >= SUB and even G[1]: the T x T tile has T*T/2 point
 // pairs (y, x), (y, x + 1), x even; thread t owns pair t % (T*T/2) for batch entries
@@ -970,49 +974,61 @@ __device__ __forceinline__ void reduce_pairs_3d(float2* __restrict__ grid, const
   float4 acc[RB];
 #pragma unroll
   for (int b = 0; b < RB; ++b) acc[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+  auto add = [&](const float4 (&v)[RB]) {
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+      acc[b].x += v[b].x;
+      acc[b].y += v[b].y;
+      acc[b].z += v[b].z;
+      acc[b].w += v[b].w;
+    }
+  };
+  auto load = [&](int it, int e, float4 (&v)[RB]) {
+    const float2* slot = sbase + ((long long)(it - g.item_base) * g.nbatch + b0) * TS::SUBN + e;
+#pragma unroll
+    for (int b = 0; b < RB; ++b)
+      v[b] = b < nb ? *(const float4*)(slot + (long long)b * TS::SUBN) : make_float4(0.f, 0.f, 0.f, 0.f);
+  };
+  // Covering tiles in row-major order, each tile's items in order (the sums' order). The
+  // first items of two neighbouring tiles along dim 2 are loaded together, further items
+  // (split hot tiles) two at a time, so that a block does not wait out one load per item.
   for (int k0 = 0; k0 < ncand[0]; ++k0)
     for (int k1 = 0; k1 < ncand[1]; ++k1)
-      for (int k2 = 0; k2 < ncand[2]; ++k2) {
-        const int c0 = cand[0][k0], c1 = cand[1][k1], c2 = cand[2][k2];
-        const int l0 = pmod(gc[0] - c0 * T, g.G[0]), l1 = pmod(gc[1] - c1 * T, g.G[1]);
-        const int l2 = pmod(gc[2] - c2 * T, g.G[2]);  // even, so l2 + 1 < SUB when l2 < SUB
-        const bool ok = inside && l0 < TS::SUB && l1 < TS::SUB && l2 < TS::SUB;
-        const int e = (l0 * TS::SUB + l1) * TS::SUB + l2;
-        const int stile = (c0 * g.ntile[1] + c1) * g.ntile[2] + c2;
-        const int i1 = titems[stile + 1];
-        int it = titems[stile];
-        if (ok)  // two items' loads at a time (split hot tiles); sums in item order
-          for (; it + 2 <= i1; it += 2) {
-            float4 v[2][RB];
+      for (int k2 = 0; k2 < ncand[2]; k2 += 2) {
+        int e[2], i0[2], i1[2];
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-              const float2* slot = sbase + ((long long)(it + j - g.item_base) * g.nbatch + b0) * TS::SUBN + e;
+        for (int j = 0; j < 2; ++j) {
+          const bool valid = k2 + j < ncand[2];
+          const int c0 = cand[0][k0], c1 = cand[1][k1], c2 = cand[2][valid ? k2 + j : k2];
+          const int l0 = pmod(gc[0] - c0 * T, g.G[0]), l1 = pmod(gc[1] - c1 * T, g.G[1]);
+          const int l2 = pmod(gc[2] - c2 * T, g.G[2]);  // even, so l2 + 1 < SUB when l2 < SUB
+          const bool ok = valid && inside && l0 < TS::SUB && l1 < TS::SUB && l2 < TS::SUB;
+          e[j] = (l0 * TS::SUB + l1) * TS::SUB + l2;
+          const int stile = (c0 * g.ntile[1] + c1) * g.ntile[2] + c2;
+          i0[j] = ok ? titems[stile] : 0;
+          i1[j] = ok ? titems[stile + 1] : 0;
+        }
+        float4 first[2][RB];
 #pragma unroll
-              for (int b = 0; b < RB; ++b)
-                v[j][b] = b < nb ? *(const float4*)(slot + (long long)b * TS::SUBN) : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int j = 0; j < 2; ++j)
+          if (i0[j] < i1[j]) load(i0[j], e[j], first[j]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          if (i0[j] >= i1[j]) continue;
+          add(first[j]);
+          int it = i0[j] + 1;
+          if constexpr (kReduceAhead3 >= 2)
+            for (; it + 2 <= i1[j]; it += 2) {
+              float4 v0[RB], v1[RB];
+              load(it, e[j], v0);
+              load(it + 1, e[j], v1);
+              add(v0);
+              add(v1);
             }
-#pragma unroll
-            for (int j = 0; j < 2; ++j)
-#pragma unroll
-              for (int b = 0; b < RB; ++b) {
-                acc[b].x += v[j][b].x;
-                acc[b].y += v[j][b].y;
-                acc[b].z += v[j][b].z;
-                acc[b].w += v[j][b].w;
-              }
-          }
-        for (; it < i1; ++it) {
-          const float2* slot = sbase + ((long long)(it - g.item_base) * g.nbatch + b0) * TS::SUBN + e;
-          if (ok) {
-#pragma unroll
-            for (int b = 0; b < RB; ++b)
-              if (b < nb) {
-                const float4 v = *(const float4*)(slot + (long long)b * TS::SUBN);
-                acc[b].x += v.x;
-                acc[b].y += v.y;
-                acc[b].z += v.z;
-                acc[b].w += v.w;
-              }
+          for (; it < i1[j]; ++it) {
+            float4 v0[RB];
+            load(it, e[j], v0);
+            add(v0);
           }
         }
       }
