@@ -281,13 +281,13 @@ def measure_grad(steps, warmup):
         x = xin.clone().requires_grad_(True)
         k = kin.clone().requires_grad_(True)
         dc = torch.sum(torch.abs(diff.sense_forward(x, k, mps, nf) - y) ** 2)
-        rec = diff.unrolled_cg_sense(y, k, mps, nf, num_iters=n_it, lamda_l2=lam, max_eigen=eig, checkpoint=True)
+        rec = diff.unrolled_cg_sense(y, k, mps, nf, num_iters=n_it, lamda_l2=lam, max_eigen=eig)
         loss = dc + torch.sum(torch.abs(rec - x0) ** 2)
         loss.backward()
         return loss.detach(), x.grad, k.grad
 
     def plain(n_it=iters):
-        """Per-coil differentiable NUFFTs composed in Python, unrolled CG without checkpointing."""
+        """Per-coil differentiable NUFFTs composed in Python, unrolled CG as torch ops."""
         x = xin.clone().requires_grad_(True)
         k = kin.clone().requires_grad_(True)
 
@@ -350,9 +350,9 @@ def measure_grad(steps, warmup):
         "dtype": "c64", "data": "synthetic (seeded spiral, phantom with random phase, Gaussian-bump coil maps)",
         "config": {"workload": "dL/dx and dL/dk of ||A(k)x-y||^2 + ||CG10(A(k),y)-x0||^2", "image": [N, N],
                    "coils": C, "interleaves": 64, "samples_per_interleave": 2048, "width": 6, "oversamp": 1.25,
-                   "cg_iters": iters, "checkpoint": True},
+                   "cg_iters": iters, "checkpoint": False},
         "plain_path": {"value": 1.0 / t_p, "unit": "evals/s", "ms": t_p * 1e3,
-                       "path": "per-coil differentiable NUFFTs, Python-composed SENSE and CG, no checkpointing"},
+                       "path": "per-coil differentiable NUFFTs, Python-composed SENSE and CG"},
         "speedup_vs_plain": t_p / t_f,
         "agreement": {"loss": abs(float(lf - lp)) / abs(float(lp)), "grad_x": rel(gxf, gxp),
                       "grad_k": rel(gkf, gkp),

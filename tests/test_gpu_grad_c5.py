@@ -12,8 +12,9 @@ defined in tests/golden/c5_problem.py. Checked against
     1.5 relative L2, so no fp32 implementation can be held to FP64 there);
   * the FP64 oracle's own gradients of the data term: dL/dx = 2 A^H (A x - y) and dL/dk by
     autograd through the oracle's pure-torch NUFFT (NaN-safe masking).
-Both the fused path (SENSE autograd functions, checkpointed unrolled CG) and the plain path
-(per-coil differentiable NUFFTs composed in Python, no checkpointing) are checked.
+Both the fused path (SENSE autograd functions, unrolled CG; the checkpointed variant must give
+the same bits) and the plain path (per-coil differentiable NUFFTs composed in Python) are
+checked.
 """
 
 import json
@@ -58,7 +59,7 @@ def c5(dev):
     return inp, fix
 
 
-def _fast(inp, dev, iters, data_only=False):
+def _fast(inp, dev, iters, data_only=False, checkpoint=False):
     from mr_recon_amd import diff
     from mr_recon_amd.fourier import hip_nufft
 
@@ -69,7 +70,7 @@ def _fast(inp, dev, iters, data_only=False):
     loss = torch.sum(torch.abs(diff.sense_forward(x, k, mps, nf) - y) ** 2)
     if not data_only:
         rec = diff.unrolled_cg_sense(y, k, mps, nf, num_iters=iters, lamda_l2=P.LAMDA, max_eigen=P.MAX_EIGEN,
-                                     checkpoint=True)
+                                     checkpoint=checkpoint)
         loss = loss + torch.sum(torch.abs(rec - x0) ** 2)
     loss.backward()
     return float(loss), x.grad.cpu(), k.grad.cpu()
@@ -147,9 +148,13 @@ def _check_trajectory(tag, loss, gk, inp, fix):
 
 def test_c5_fused_path_vs_fp64_finite_differences(dev, c5):
     inp, fix = c5
-    _say("C5: fused path (SENSE autograd + checkpointed unrolled CG), 256^2 x 16 coils, 64 x 2048")
-    loss, gx, _ = _fast(inp, dev, P.ITERS)
+    _say("C5: fused path (SENSE autograd + unrolled CG), 256^2 x 16 coils, 64 x 2048")
+    loss, gx, gk10 = _fast(inp, dev, P.ITERS)
     _check_loss_and_image("fused", loss, gx, fix)
+    # the checkpointed CG recomputes each iteration in backward: the same operations, the same bits
+    loss_ck, gx_ck, gk_ck = _fast(inp, dev, P.ITERS, checkpoint=True)
+    _say(f"C5: checkpointed CG: loss {loss_ck:.10e} vs {loss:.10e}")
+    assert loss_ck == loss and torch.equal(gx_ck, gx) and torch.equal(gk_ck, gk10)
     loss, _, gk = _fast(inp, dev, fix["k_iters"])
     _check_trajectory("fused", loss, gk, inp, fix)
 
