@@ -245,6 +245,18 @@ int mra_cg_init(const void* b, const void* ax, const void* x, void* r, void* p, 
                 double* part, double* sc, void* stream);
 int mra_cg_step(void* x, void* r, void* p, const void* ap, long long n, float lam, double* part, double* sc,
                 void* stream);
+/* One CG step as a differentiable function, for unrolling CG under autograd: forward out of
+ * place from nrm = (A^H A) p: ap = inv * nrm + lam * p, pAp, alpha, x1, r1, rz1 = <r1, r1>,
+ * beta, p1 (FP64 dots; sd = double[8]: rz, pAp, rz1, beta, alpha), and the adjoint of that
+ * step for the cotangents of (x1, r1, p1, rz1), any of which may be null. The reference has
+ * no fused counterpart: its CG (algs.py:779-872) is differentiated op by op by torch
+ * autograd. part: mra_cg_partials() doubles (forward), 4 x that (backward). */
+int mra_cg_step_fwd(const void* x, const void* r, const void* p, const void* nrm, const double* rz, long long n,
+                    float inv, float lam, void* x1, void* r1, void* p1, double* rz1, double* sd, double* part,
+                    void* stream);
+int mra_cg_step_bwd(const void* p, const void* nrm, const void* r1, const double* sd, const void* gx1,
+                    const void* gr1, const void* gp1, const double* grz1, long long n, float inv, float lam,
+                    void* gr, void* gp, void* gnrm, double* grz, double* part, double* bs, void* stream);
 /* nseg independent systems of seg_len elements (batched problems; the reference runs
  * one conjugate_gradient per problem). */
 int mra_cg_seg_blocks(void);

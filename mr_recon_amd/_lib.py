@@ -78,6 +78,9 @@ SIGNATURES = {
     "mra_cdot_re": (_i, [_vp, _vp, _ll, _f, _vp, _vp, _i, _vp]),
     "mra_cg_init": (_i, [_vp, _vp, _vp, _vp, _vp, _ll, _f, _vp, _vp, _vp]),
     "mra_cg_step": (_i, [_vp, _vp, _vp, _vp, _ll, _f, _vp, _vp, _vp]),
+    "mra_cg_step_fwd": (_i, [_vp, _vp, _vp, _vp, _vp, _ll, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mra_cg_step_bwd": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _ll, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp,
+                             _vp]),
     "mra_cg_init_seg": (_i, [_vp, _vp, _vp, _vp, _vp, _ll, _i, _f, _vp, _vp, _vp]),
     "mra_cg_step_seg": (_i, [_vp, _vp, _vp, _vp, _ll, _i, _f, _vp, _vp, _vp]),
     "mra_nn_gather": (_i, [_vp, _ll, _vp, _ll, _i, _f, _vp, _vp]),

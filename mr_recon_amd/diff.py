@@ -227,17 +227,64 @@ def dot_re64(a, b):
     return _DotRe64.apply(a, b)
 
 
+def _vec(t: torch.Tensor) -> torch.Tensor:
+    """A complex64 vector the kernels can read: conj/neg views materialised, contiguous."""
+    return t.to(complex_dtype).resolve_conj().resolve_neg().contiguous()
+
+
+class _CGStepFn(Function):
+    """One CG iteration (algs.py:779-872 with P = identity) once nrm = (A^H D A) p is formed:
+    ap = inv * nrm + lam * p, pAp = Re<p, ap>, alpha = rz / pAp, x1 = x + alpha p,
+    r1 = r - alpha ap, rz1 = <r1, r1>, p1 = r1 + (rz1 / rz) p, with the inner products and
+    step sizes in FP64 (alpha and beta cast to fp32 for the vector updates). Forward and
+    backward are a handful of fused kernels (``mra_cg_step_fwd`` / ``mra_cg_step_bwd``)
+    instead of a dozen torch ops, each with its own autograd node and scalar reductions."""
+
+    @staticmethod
+    def forward(ctx, x, r, p, rz, nrm, inv, lam):
+        x, r, p, nrm = _vec(x), _vec(r), _vec(p), _vec(nrm)
+        rz = rz.to(torch.float64).contiguous()
+        dev = x.device
+        L = _lib.lib()
+        x1, r1, p1 = torch.empty_like(x), torch.empty_like(r), torch.empty_like(p)
+        rz1 = torch.empty((), dtype=torch.float64, device=dev)
+        sd = torch.empty(8, dtype=torch.float64, device=dev)
+        part = torch.empty(L.mra_cg_partials(), dtype=torch.float64, device=dev)
+        _lib.check(L.mra_cg_step_fwd(_lib.ptr(x), _lib.ptr(r), _lib.ptr(p), _lib.ptr(nrm), _lib.ptr(rz), x.numel(),
+                                     float(inv), float(lam), _lib.ptr(x1), _lib.ptr(r1), _lib.ptr(p1),
+                                     _lib.ptr(rz1), _lib.ptr(sd), _lib.ptr(part), _lib.stream_ptr(dev)))
+        ctx.save_for_backward(p, nrm, r1, sd)
+        ctx.inv, ctx.lam = float(inv), float(lam)
+        return x1, r1, p1, rz1
+
+    @staticmethod
+    def backward(ctx, gx1, gr1, gp1, grz1):
+        p, nrm, r1, sd = ctx.saved_tensors
+        dev = p.device
+        L = _lib.lib()
+        gx1, gr1, gp1 = (None if g is None else _vec(g) for g in (gx1, gr1, gp1))
+        grz1 = None if grz1 is None else grz1.to(torch.float64).contiguous()
+        gr, gp, gnrm = torch.empty_like(p), torch.empty_like(p), torch.empty_like(p)
+        grz = torch.empty((), dtype=torch.float64, device=dev)
+        part = torch.empty(4 * L.mra_cg_partials(), dtype=torch.float64, device=dev)
+        bs = torch.empty(8, dtype=torch.float64, device=dev)
+        _lib.check(L.mra_cg_step_bwd(_lib.ptr(p), _lib.ptr(nrm), _lib.ptr(r1), _lib.ptr(sd), _lib.ptr(gx1),
+                                     _lib.ptr(gr1), _lib.ptr(gp1), _lib.ptr(grz1), p.numel(), ctx.inv, ctx.lam,
+                                     _lib.ptr(gr), _lib.ptr(gp), _lib.ptr(gnrm), _lib.ptr(grz), _lib.ptr(part),
+                                     _lib.ptr(bs), _lib.stream_ptr(dev)))
+        return gx1, gr, gp, grz, gnrm, None, None
+
+
+def cg_step(x, r, p, rz, nrm, inv_eig: float, lamda: float):
+    """Differentiable CG update (x, r, p, rz) -> (x1, r1, p1, rz1) given nrm = (A^H D A) p;
+    see ``_CGStepFn``."""
+    return _CGStepFn.apply(x, r, p, rz, nrm, inv_eig, lamda)
+
+
 def _cg_iter(x, r, p, rz, trj, mps, nufft, dcf, lamda, inv_eig):
-    """One CG step (algs.py:779-872 with P = identity) as differentiable torch ops; the
-    inner products and the step sizes alpha, beta are FP64 scalars."""
-    ap = sense_normal(p, trj, mps, nufft, dcf) * inv_eig + lamda * p
-    pap = dot_re64(p, ap)
-    alpha = (rz / pap).to(torch.float32)
-    x = x + alpha * p
-    r = r - alpha * ap
-    rz_new = dot_re64(r, r)
-    p = r + (rz_new / rz).to(torch.float32) * p
-    return x, r, p, rz_new
+    """One CG step (algs.py:779-872 with P = identity): the normal operator as its autograd
+    function, the vector work and its FP64 scalars as one fused differentiable step."""
+    return cg_step(x, r, p, rz, sense_normal(p, trj, mps, nufft, dcf), inv_eig, lamda)
 
 
 def unrolled_cg_sense(ksp, trj, mps, nufft: hip_nufft, num_iters: int = 10, lamda_l2: float = 0.0,
@@ -270,4 +317,4 @@ def unrolled_cg_sense(ksp, trj, mps, nufft: hip_nufft, num_iters: int = 10, lamd
 
 
 __all__ = ["SenseForwardFn", "SenseAdjointFn", "SenseNormalFn", "nufft_forward", "nufft_adjoint", "sense_forward",
-           "sense_adjoint", "sense_normal", "unrolled_cg_sense", "dot_re64"]
+           "sense_adjoint", "sense_normal", "unrolled_cg_sense", "dot_re64", "cg_step"]

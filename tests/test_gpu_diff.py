@@ -210,3 +210,96 @@ def test_normal_function_matches_composition(dev, dim, use_dcf):
     assert rel_l2(o1, o2) < 1e-6
     assert rel_l2(gx1, gx2) < 1e-6
     assert rel_l2(gk1, gk2) < 1e-6
+
+
+def _cg_step_torch(x, r, p, rz, nrm, inv, lam):
+    """The CG update as the composition of torch ops (algs.py:779-872, P = identity): the
+    closed form the fused step is differentiated against, run in complex128 on the CPU."""
+    ap = nrm * inv + lam * p
+    pap = torch.real(torch.sum(p.conj() * ap))
+    alpha = rz / pap
+    x1 = x + alpha * p
+    r1 = r - alpha * ap
+    rz1 = torch.real(torch.sum(r1.conj() * r1))
+    p1 = r1 + (rz1 / rz) * p
+    return x1, r1, p1, rz1
+
+
+def _cg_step_torch32(x, r, p, rz, nrm, inv, lam):
+    """The same composition with complex64 vectors and FP64 inner products and step sizes
+    (the torch ops the fused step replaces): the fp32 yardstick of the tolerance."""
+    ap = nrm * inv + lam * p
+    pap = torch.real(torch.sum(p.conj() * ap, dtype=torch.complex128))
+    alpha = (rz / pap).to(torch.float32)
+    x1 = x + alpha * p
+    r1 = r - alpha * ap
+    rz1 = torch.real(torch.sum(r1.conj() * r1, dtype=torch.complex128))
+    p1 = r1 + (rz1 / rz).to(torch.float32) * p
+    return x1, r1, p1, rz1
+
+
+@pytest.mark.parametrize("which", ["all", "x_only", "rz_only"])
+def test_cg_step_fused_vs_fp64_composition(dev, which):
+    """cg_step (fused kernels, fp32 vectors, FP64 scalars) against the complex128 autograd of
+    the torch composition: outputs and the gradients of a random linear functional of them
+    with respect to x, r, p, rz and nrm, with some cotangents absent. Bound per gradient:
+    max(2e-5, 5x the error of the fp32 torch composition itself; the rz gradient goes through
+    Re<r1, ap>, a near-cancelling sum of fp32-rounded r1)."""
+    from mr_recon_amd.diff import cg_step
+
+    g = torch.Generator().manual_seed(11)
+
+    def rnd(*shape):
+        return torch.complex(torch.randn(shape, generator=g), torch.randn(shape, generator=g)).to(torch.complex64)
+
+    n = (48, 40)
+    x, r, p, nrm = rnd(*n), rnd(*n), rnd(*n), rnd(*n)
+    p = r + 0.3 * p  # a realistic search direction: pAp > 0 below
+    nrm = nrm * 0.1 + 2.0 * p  # a positive-definite-like response
+    inv, lam = 0.7, 0.05
+    rz = torch.real(torch.sum(r.conj().to(torch.complex128) * r.to(torch.complex128)))
+    cot = [rnd(*n), rnd(*n), rnd(*n), torch.tensor(0.37, dtype=torch.float64)]
+    if which == "x_only":
+        cot = [cot[0], None, None, None]
+    elif which == "rz_only":
+        cot = [None, None, None, cot[3]]
+
+    def functional(outs):
+        tot = 0.0
+        for o, c in zip(outs, cot):
+            if c is None:
+                continue
+            if o.is_complex():
+                tot = tot + torch.real(torch.sum(c.to(o.device, o.dtype).conj() * o))
+            else:
+                tot = tot + (c.to(o.device) * o)
+        return tot
+
+    ins64 = [t.to(torch.complex128).requires_grad_(True) for t in (x, r, p)]
+    rz64 = rz.clone().requires_grad_(True)
+    nrm64 = nrm.to(torch.complex128).requires_grad_(True)
+    out64 = _cg_step_torch(*ins64, rz64, nrm64, inv, lam)
+    grads64 = torch.autograd.grad(functional(out64), [*ins64, rz64, nrm64], allow_unused=True)
+
+    insd = [t.to(dev).requires_grad_(True) for t in (x, r, p)]
+    rzd = rz.to(dev).requires_grad_(True)
+    nrmd = nrm.to(dev).requires_grad_(True)
+    outd = cg_step(*insd, rzd, nrmd, inv, lam)
+    for a, b in zip(outd, out64):
+        assert rel_l2(a.detach().cpu().to(b.dtype), b.detach()) < 1e-6
+    gradsd = torch.autograd.grad(functional(outd), [*insd, rzd, nrmd], allow_unused=True)
+    ins32 = [t.clone().requires_grad_(True) for t in (x, r, p)]
+    rz32 = rz.clone().requires_grad_(True)
+    nrm32 = nrm.clone().requires_grad_(True)
+    grads32 = torch.autograd.grad(functional(_cg_step_torch32(*ins32, rz32, nrm32, inv, lam)),
+                                  [*ins32, rz32, nrm32], allow_unused=True)
+
+    def err(a, b):
+        return float(torch.linalg.vector_norm(a.detach().cpu().to(b.dtype) - b) / torch.linalg.vector_norm(b))
+
+    for name, a, b, c in zip(["x", "r", "p", "rz", "nrm"], gradsd, grads64, grads32):
+        if b is None or not torch.any(b != 0):
+            assert a is None or float(torch.linalg.vector_norm(a.detach().cpu().to(torch.complex128))) == 0.0, name
+            continue
+        bound = max(2e-5, 5 * err(c, b))
+        assert err(a, b) < bound, (name, err(a, b), bound)
