@@ -13,9 +13,8 @@ gather (``mra_interp_grad``):
      dL/dt_m = sum_c Re( conj(dcf_m y_cm) * sum_p dW_p/dt * H_c[p] ) / w^d,  H_c = FFT(pad(apod S_c gx))/sqrt(prod N)
 
 with t = k * (ceil(s N)/N) + ceil(s N)//2, so dL/dk = dL/dt * ceil(s N)/N. The
-oversampled grids F_c / H_c are recomputed in backward (one pad + FFT), not
-saved: a checkpointed recompute that keeps the forward's memory at the size of
-its output. ``unrolled_cg_sense`` runs CG-SENSE iterations as differentiable
+forward's oversampled grids F_c are kept for backward up to KEEP_GRIDS_BYTES and
+recomputed (one pad + FFT) beyond; H_c is formed in backward. ``unrolled_cg_sense`` runs CG-SENSE iterations as differentiable
 torch ops around these operators, each iteration optionally under
 ``torch.utils.checkpoint``.
 
@@ -34,6 +33,11 @@ from torch.autograd import Function
 from . import _lib, ops
 from .dtypes import complex_dtype
 from .fourier import hip_nufft
+
+
+# SenseNormalFn keeps the forward's oversampled coil grids for the trajectory gradient up to
+# this size (one copy per application kept by autograd, e.g. per unrolled CG iteration)
+KEEP_GRIDS_BYTES = 256 << 20
 
 
 class _Ctx:
@@ -92,15 +96,17 @@ class SenseForwardFn(Function):
     @staticmethod
     def forward(ctx, x, trj, nufft, mps):
         c = _Ctx(nufft, trj, mps, None)
-        y = c.forward(x.detach())
+        grids = c.grids(x.detach())
+        y = c.gather(grids)
         ctx.c = c
-        ctx.save_for_backward(x.detach())
+        keep = ctx.needs_input_grad[1] and grids.numel() * grids.element_size() <= KEEP_GRIDS_BYTES
+        ctx.save_for_backward(x.detach(), grids if keep else None)
         return y.reshape(c.C, *c.trj_shape)
 
     @staticmethod
     def backward(ctx, gy):
         c = ctx.c
-        (x,) = ctx.saved_tensors
+        x, grids_x = ctx.saved_tensors
         gx = gt = None
         gy = gy.to(complex_dtype).contiguous()
         if ctx.needs_input_grad[0]:
@@ -108,7 +114,8 @@ class SenseForwardFn(Function):
             if not torch.is_complex(x):
                 gx = gx.real
         if ctx.needs_input_grad[1]:
-            gt = c.dk(c.grids(x), gy)  # checkpointed recompute of the forward grids
+            # the forward's grids, or their recompute when they were too large to keep
+            gt = c.dk(grids_x if grids_x is not None else c.grids(x), gy)
         return gx, gt, None, None
 
 
@@ -143,7 +150,8 @@ class SenseAdjointFn(Function):
 
 class SenseNormalFn(Function):
     """(x (*im), trj) -> A^H D A x in one function, the k-space in between kept in plan
-    (sorted) order so the gather stores it and gridding loads it contiguously. Backward
+    (sorted) order so the gather stores it and gridding loads it contiguously; G_x (the
+    oversampled grids of x) is kept for backward up to KEEP_GRIDS_BYTES. Backward
     (D real, A^H D A self-adjoint), with g the output cotangent and G_g = FFT(pad(S g)):
         gx = A^H D A g
         gt = dk(G_g, D A x) + dk(G_x, D A g)      (the adjoint's and the forward's k-terms)
@@ -153,16 +161,19 @@ class SenseNormalFn(Function):
     @staticmethod
     def forward(ctx, x, trj, nufft, mps, dcf):
         c = _Ctx(nufft, trj, mps, dcf)
-        y = c.gather(c.grids(x.detach()), plan_order=True)
+        grids = c.grids(x.detach())
+        y = c.gather(grids, plan_order=True)
         out = c.adjoint(y, c.dcf_po, plan_order=True)
         ctx.c = c
-        ctx.save_for_backward(x.detach(), y)
+        # the trajectory gradient needs G_x again: kept when small, else recomputed
+        keep = ctx.needs_input_grad[1] and grids.numel() * grids.element_size() <= KEEP_GRIDS_BYTES
+        ctx.save_for_backward(x.detach(), y, grids if keep else None)
         return out
 
     @staticmethod
     def backward(ctx, gout):
         c = ctx.c
-        x, y = ctx.saved_tensors
+        x, y, grids_x = ctx.saved_tensors
         gout = gout.to(complex_dtype).contiguous()
         gx = gt = None
         need_x, need_t = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
@@ -176,7 +187,8 @@ class SenseNormalFn(Function):
                     gx = gx.real
             if need_t:
                 dy = y * c.dcf_po if c.dcf_po is not None else y
-                gt = c.dk(Gg, dy, plan_order=True) + c.dk(c.grids(x), dag, plan_order=True)
+                Gx = grids_x if grids_x is not None else c.grids(x)
+                gt = c.dk(Gg, dy, plan_order=True) + c.dk(Gx, dag, plan_order=True)
         return gx, gt, None, None, None
 
 
