@@ -303,3 +303,51 @@ def test_cg_step_fused_vs_fp64_composition(dev, which):
             continue
         bound = max(2e-5, 5 * err(c, b))
         assert err(a, b) < bound, (name, err(a, b), bound)
+
+
+@pytest.mark.parametrize("fn", ["forward", "normal"])
+def test_kept_and_recomputed_forward_grids_same_bits(dev, monkeypatch, fn):
+    """The SENSE functions keep their forward's oversampled grids for the trajectory gradient
+    up to KEEP_GRIDS_BYTES and recompute them beyond: both branches give the same bits."""
+    from mr_recon_amd import diff
+    from mr_recon_amd.fourier import hip_nufft
+
+    trj, mps, x, y = _problem(seed=13)
+    nf = hip_nufft(x.shape, oversamp=1.25, width=6)
+    md, yd = mps.to(dev), y.to(dev)
+
+    def run():
+        xd = x.to(dev).requires_grad_(True)
+        kd = trj.to(dev).requires_grad_(True)
+        if fn == "forward":
+            loss = torch.sum(torch.abs(diff.sense_forward(xd, kd, md, nf) - yd) ** 2)
+        else:
+            loss = torch.sum(torch.abs(diff.sense_normal(xd, kd, md, nf)) ** 2)
+        loss.backward()
+        return xd.grad.clone(), kd.grad.clone()
+
+    gx_keep, gk_keep = run()
+    monkeypatch.setattr(diff, "KEEP_GRIDS_BYTES", 0)
+    gx_re, gk_re = run()
+    assert torch.equal(gx_keep, gx_re) and torch.equal(gk_keep, gk_re)
+
+
+def test_cg_step_zero_direction_stays_finite(dev):
+    """A search direction of zeros (pAp = 0, a converged CG) gives alpha = 0 instead of 0/0:
+    the iterate and residual stay put, p becomes beta * 0 + r, and the gradients are finite."""
+    from mr_recon_amd.diff import cg_step
+
+    g = torch.Generator().manual_seed(3)
+    n = (16, 24)
+    x = torch.randn(n, generator=g).to(torch.complex64).to(dev).requires_grad_(True)
+    r = torch.randn(n, generator=g).to(torch.complex64).to(dev).requires_grad_(True)
+    p = torch.zeros(n, dtype=torch.complex64, device=dev, requires_grad=True)
+    nrm = torch.zeros(n, dtype=torch.complex64, device=dev, requires_grad=True)
+    rz = torch.real(torch.sum(r.detach().conj().to(torch.complex128) * r.detach())).requires_grad_(True)
+    x1, r1, p1, rz1 = cg_step(x, r, p, rz, nrm, 0.5, 0.01)
+    assert torch.equal(x1, x) and torch.equal(r1, r) and torch.equal(p1, r)
+    assert abs(float(rz1) - float(rz)) <= 1e-12 * float(rz)
+    loss = torch.sum(torch.abs(x1) ** 2) + torch.sum(torch.abs(p1) ** 2) + rz1
+    loss.backward()
+    for t in (x, r, p, nrm, rz):
+        assert torch.all(torch.isfinite(torch.view_as_real(t.grad) if t.grad.is_complex() else t.grad))
