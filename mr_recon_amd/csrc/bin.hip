@@ -106,9 +106,11 @@ __global__ __launch_bounds__(256) void pass_hist_kernel(const int* __restrict__ 
   for (int k = threadIdx.x; k < mod; k += blockDim.x) hist[(long long)k * nchunks + blockIdx.x] = sh[k];
 }
 
-// Stable scatter of one pass: one wave per chunk walks its positions in order, 64
-// at a time; lanes sharing a digit are ranked by lane order, so equal digits keep
-// their input order. Moves the sample index (perm) and the carried tile id.
+// Stable scatter of one pass: one wave per chunk walks its positions in order, 64 at a
+// time. Each lane finds the lanes holding its digit with one ballot per digit bit (the
+// lanes whose digits agree on every bit), ranks itself among them by lane order, and the
+// lowest of them advances the digit's cursor: equal digits keep their input order. Moves
+// the sample index (perm) and the carried tile id.
 __global__ __launch_bounds__(64) void pass_scatter_kernel(const int* __restrict__ key, long long npts, int div,
                                                           int mod, const int* __restrict__ scanned, int nchunks,
                                                           int chunk_len, const int* __restrict__ perm_in,
@@ -122,28 +124,26 @@ __global__ __launch_bounds__(64) void pass_scatter_kernel(const int* __restrict_
   const long long c0 = (long long)blockIdx.x * chunk_len;
   const long long c1 = min(c0 + chunk_len, npts);
   const unsigned long long lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  const int nbits = mod > 1 ? 32 - __clz(mod - 1) : 0;  // bits of the largest digit
   for (long long b = c0; b < c1; b += 64) {
     const long long i = b + lane;
     const bool valid = i < c1;
-    const int dig = valid ? digit_of(key[i], div, mod) : -1;
-    unsigned long long active = __ballot(valid);
-    while (active) {
-      const int leader = __ffsll((long long)active) - 1;
-      const int ldig = __shfl(dig, leader);
-      const unsigned long long mask = __ballot(dig == ldig) & active;
-      int c = 0;
-      if (lane == leader) {
-        c = cursor[ldig];
-        if (c < 0) c = scanned[(long long)ldig * nchunks + blockIdx.x];
-        cursor[ldig] = c + __popcll(mask);
-      }
-      c = __shfl(c, leader);
-      if ((mask >> lane) & 1ull) {
-        const long long pos = c + __popcll(mask & lt_mask);
-        perm_out[pos] = perm_in ? perm_in[i] : (int)i;
-        carry_out[pos] = carry_in[i];
-      }
-      active &= ~mask;
+    const int dig = valid ? digit_of(key[i], div, mod) : 0;
+    unsigned long long same = __ballot(valid);
+    for (int bit = 0; bit < nbits; ++bit) {
+      const bool set = ((dig >> bit) & 1) != 0;
+      const unsigned long long bal = __ballot(set);
+      same &= set ? bal : ~bal;
+    }
+    if (valid) {
+      // one wave's LDS operations complete in order: every lane reads the cursor before the
+      // lowest lane of its digit writes the advanced one
+      int c = cursor[dig];
+      if (c < 0) c = scanned[(long long)dig * nchunks + blockIdx.x];
+      const long long pos = c + __popcll(same & lt_mask);
+      perm_out[pos] = perm_in ? perm_in[i] : (int)i;
+      carry_out[pos] = carry_in[i];
+      if ((same & lt_mask) == 0ull) cursor[dig] = c + __popcll(same);
     }
   }
 }

@@ -31,12 +31,13 @@ MAX_DIGIT = 8192  # largest radix of one sort pass (mra_bin_build)
 def binning_chunk(npts: int, ntiles: int, tile: int, ndim: int) -> int:
     """Samples per radix-sort chunk. One wave walks a chunk in order, so the chunks are the
     parallelism of each scatter pass (4096-sample chunks left a 131 k-sample trajectory 32
-    waves); each pass's histogram holds digits x chunks counts. Aim for about 4096 chunks of
-    128..4096 samples, with at most 2^25 counts per histogram."""
+    waves); each pass's histogram holds digits x chunks counts, written and scanned whole.
+    Aim for about 1024 chunks of 128..4096 samples, with at most 2^24 counts per histogram
+    (measured: a 5.12 M-sample 3D plan takes 0.84 ms at 4096, 1.4 ms at 1024, 7 ms at 128)."""
     digits = max(tile ** ndim, min(ntiles + 1, MAX_DIGIT))
-    c = -(-npts // 4096)
+    c = -(-npts // 1024)
     c = min(max(-(-c // 64) * 64, 128), 4096)
-    while c < 4096 and digits * -(-npts // c) > 1 << 25:
+    while c < 4096 and digits * -(-npts // c) > 1 << 24:
         c *= 2
     return min(c, 4096)
 
