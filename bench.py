@@ -358,7 +358,11 @@ def measure_grad(steps, warmup):
                       "grad_k": rel(gkf, gkp),
                       "cg_iters_3": {"loss": abs(float(l3f - l3p)) / abs(float(l3p)), "grad_k": rel(gk3f, gk3p)},
                       "data_term_only": {"loss": abs(float(ldf - ldp)) / abs(float(ldp)), "grad_x": rel(gxdf, gxdp),
-                                         "grad_k": rel(gkdf, gkdp)}},
+                                         "grad_k": rel(gkdf, gkdp)},
+                      "note": "at 10 CG iterations dL/dk of any fp32 CG departs from FP64 (the fp32 oracle's is "
+                              "1.5 rel L2 away), so two fp32 paths need not agree there; cg_iters_3 compares them "
+                              "where fp32 tracks FP64, and tests/test_gpu_grad_c5.py checks both paths against "
+                              "FP64 finite differences"},
     }
     return rec
 
