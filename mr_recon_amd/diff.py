@@ -62,18 +62,21 @@ class _Ctx:
         self.s_adj = self.s_interp * self.s_fwd
         self.dt_dk = nufft.scale_table(trj.device, self.im_size)
 
-    def grids(self, x: torch.Tensor) -> torch.Tensor:
-        """FFT(pad(apod * S_c x)) / sqrt(prod N): (C, *G) in the unshifted layout."""
+    def grids(self, x: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """FFT(pad(apod * S_c x)) / sqrt(prod N): (C, *G) in the unshifted layout (new, or
+        written into ``out``)."""
         x = x.to(complex_dtype).reshape(1, *self.im_size)
-        g = ops.pad_fft(x, self.im_size, self.G, smaps=self.mps, apod=self.apod, scale=self.s_fwd)
+        g = ops.pad_fft(x, self.im_size, self.G, smaps=self.mps, apod=self.apod, scale=self.s_fwd, out=out)
         return g.reshape(self.C, *self.G)
 
     def forward(self, x):
         return self.gather(self.grids(x))
 
-    def gather(self, grids, plan_order=False):
-        """Interpolation of the oversampled grids: (C, M) in sample order, or in plan order."""
-        out = torch.empty((self.C, self.plan.npts), dtype=complex_dtype, device=grids.device)
+    def gather(self, grids, plan_order=False, out=None):
+        """Interpolation of the oversampled grids: (C, M) in sample order, or in plan order
+        (new, or written into ``out``)."""
+        if out is None:
+            out = torch.empty((self.C, self.plan.npts), dtype=complex_dtype, device=grids.device)
         geo = self.plan.geom(self.C, self.plan.npts, self.s_interp, ksp_plan_order=True) if plan_order else None
         return ops.interp(grids, self.plan, out_scale=self.s_interp, out=out, geom=geo)
 
@@ -84,8 +87,10 @@ class _Ctx:
 
     def dk(self, grids, a, plan_order=False):
         """dL/dk (M, d) from the KB-derivative gather of ``grids`` against coefficients ``a``
-        (sample order, or plan order)."""
-        gt = ops.interp_grad(grids, a.to(complex_dtype).reshape(self.C, -1).contiguous(), self.plan,
+        (sample order, or plan order), summed over their batch (C grids, or several terms'
+        grids stacked)."""
+        B = grids.shape[0]
+        gt = ops.interp_grad(grids, a.to(complex_dtype).reshape(B, -1).contiguous(), self.plan,
                              out_scale=self.s_interp, a_plan_order=plan_order)
         return (gt * self.dt_dk).reshape(*self.trj_shape, self.d)
 
@@ -156,39 +161,49 @@ class SenseNormalFn(Function):
         gx = A^H D A g
         gt = dk(G_g, D A x) + dk(G_x, D A g)      (the adjoint's and the forward's k-terms)
     G_g is computed once and serves both the gather D A g and the first k-term; A x is
-    saved from the forward."""
+    saved from the forward. When G_x is kept, G_g is written beside it and A x beside A g,
+    so the two k-terms are one KB-derivative gather over 2C grids."""
 
     @staticmethod
     def forward(ctx, x, trj, nufft, mps, dcf):
         c = _Ctx(nufft, trj, mps, dcf)
-        grids = c.grids(x.detach())
-        y = c.gather(grids, plan_order=True)
+        need_t = ctx.needs_input_grad[1]
+        gbytes = c.C * math.prod(c.G) * 8
+        keep = need_t and 2 * gbytes <= KEEP_GRIDS_BYTES
+        dev = x.device
+        # [G_g | G_x]: G_g is formed in backward
+        g2 = torch.empty((2, c.C, *c.G), dtype=complex_dtype, device=dev) if keep else None
+        grids = c.grids(x.detach(), out=g2[1] if keep else None)
+        # [A x | A g] in plan order: A g is formed in backward
+        y2 = torch.empty((2, c.C, c.plan.npts), dtype=complex_dtype, device=dev) if need_t else None
+        y = c.gather(grids, plan_order=True, out=y2[0] if need_t else None)
         out = c.adjoint(y, c.dcf_po, plan_order=True)
         ctx.c = c
-        # the trajectory gradient needs G_x again: kept when small, else recomputed
-        keep = ctx.needs_input_grad[1] and grids.numel() * grids.element_size() <= KEEP_GRIDS_BYTES
-        ctx.save_for_backward(x.detach(), y, grids if keep else None)
+        ctx.save_for_backward(x.detach(), y if y2 is None else y2, g2)
         return out
 
     @staticmethod
     def backward(ctx, gout):
         c = ctx.c
-        x, y, grids_x = ctx.saved_tensors
+        x, ys, g2 = ctx.saved_tensors
         gout = gout.to(complex_dtype).contiguous()
         gx = gt = None
         need_x, need_t = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
         if need_x or need_t:
-            Gg = c.grids(gout)
-            ag = c.gather(Gg, plan_order=True)  # A g, plan order
-            dag = ag * c.dcf_po if c.dcf_po is not None else ag
+            Gg = c.grids(gout, out=g2[0] if g2 is not None else None)
+            # A g, plan order (beside A x when the trajectory gradient is needed)
+            ag = c.gather(Gg, plan_order=True, out=ys[1] if need_t else None)
             if need_x:
                 gx = c.adjoint(ag, c.dcf_po, plan_order=True).reshape(x.shape)
                 if not torch.is_complex(x):
                     gx = gx.real
             if need_t:
-                dy = y * c.dcf_po if c.dcf_po is not None else y
-                Gx = grids_x if grids_x is not None else c.grids(x)
-                gt = c.dk(Gg, dy, plan_order=True) + c.dk(Gx, dag, plan_order=True)
+                # [D A x | D A g]: the coefficients of the adjoint's and the forward's k-terms
+                dys = ys * c.dcf_po if c.dcf_po is not None else ys
+                if g2 is not None:
+                    gt = c.dk(g2.reshape(2 * c.C, *c.G), dys.reshape(2 * c.C, -1), plan_order=True)
+                else:
+                    gt = c.dk(Gg, dys[0], plan_order=True) + c.dk(c.grids(x), dys[1], plan_order=True)
         return gx, gt, None, None, None
 
 

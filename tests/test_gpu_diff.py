@@ -308,7 +308,9 @@ def test_cg_step_fused_vs_fp64_composition(dev, which):
 @pytest.mark.parametrize("fn", ["forward", "normal"])
 def test_kept_and_recomputed_forward_grids_same_bits(dev, monkeypatch, fn):
     """The SENSE functions keep their forward's oversampled grids for the trajectory gradient
-    up to KEEP_GRIDS_BYTES and recompute them beyond: both branches give the same bits."""
+    up to KEEP_GRIDS_BYTES and recompute them beyond: the same bits for the forward function;
+    the normal function's kept branch sums its two k-terms in one gather over 2C grids (another
+    fp32 summation order), its dL/dx is bit-identical."""
     from mr_recon_amd import diff
     from mr_recon_amd.fourier import hip_nufft
 
@@ -329,7 +331,11 @@ def test_kept_and_recomputed_forward_grids_same_bits(dev, monkeypatch, fn):
     gx_keep, gk_keep = run()
     monkeypatch.setattr(diff, "KEEP_GRIDS_BYTES", 0)
     gx_re, gk_re = run()
-    assert torch.equal(gx_keep, gx_re) and torch.equal(gk_keep, gk_re)
+    assert torch.equal(gx_keep, gx_re)
+    if fn == "forward":
+        assert torch.equal(gk_keep, gk_re)
+    else:
+        assert rel_l2(gk_keep, gk_re) < 1e-6
 
 
 def test_cg_step_zero_direction_stays_finite(dev):
