@@ -46,7 +46,50 @@ def test_forward_gradients_vs_oracle_autograd(dev, os_, w):
     k64 = trj.to(torch.float64).requires_grad_(True)
     l64 = _oracle_loss(x64, k64, mps.to(torch.complex128), y.to(torch.complex128), os_, w)
     l64.backward()
-    assert abs(float(loss) - float(l64)) / float(l64) < 1e-4
+    assert abs(float(loss.detach()) - float(l64)) / float(l64) < 1e-4
+    assert rel_l2(xd.grad, x64.grad) < 1e-4
+    assert rel_l2(kd.grad.to(torch.float64), k64.grad) < 1e-3
+
+
+def test_forward_gradients_3d_vs_oracle_autograd(dev):
+    """3D SENSE forward (kooshball, 3 coils): dL/dx and dL/dk vs FP64 autograd through the
+    oracle, the normal function and the adjoint function included through one loss."""
+    from mr_recon_amd.diff import sense_adjoint, sense_forward, sense_normal
+    from mr_recon_amd.fourier import hip_nufft
+
+    im, C = (12, 12, 12), 3
+    trj = synth.kooshball_3d(24, 20, 12).reshape(-1, 3)
+    g = torch.Generator().manual_seed(21)
+    trj = trj + 0.2 * (torch.rand(trj.shape, generator=g) - 0.5)
+    mps = synth.coil_maps(C, im, seed=4)
+    x = synth.random_image(im, seed=5)
+    y = synth.random_image((C, trj.shape[0]), seed=6) * 0.1
+    nf = hip_nufft(im, oversamp=1.25, width=4)
+    xd = x.to(dev).requires_grad_(True)
+    kd = trj.to(dev).requires_grad_(True)
+    md = mps.to(dev)
+    loss = torch.sum(torch.abs(sense_forward(xd, kd, md, nf) - y.to(dev)) ** 2)
+    # the normal and adjoint functions in the same loss: their k-terms add to dL/dk
+    loss = loss + 1e-3 * torch.sum(torch.abs(sense_normal(xd, kd, md, nf)) ** 2)
+    loss = loss + 1e-2 * torch.sum(torch.abs(sense_adjoint(y.to(dev), kd, md, nf)) ** 2)
+    loss.backward()
+
+    ref = TritonNufftRef(im, oversamp=1.25, width=4, dtype=torch.float64, coords64=True, safe=True)
+    m64 = mps.to(torch.complex128)
+    x64 = x.to(torch.complex128).requires_grad_(True)
+    k64 = trj.to(torch.float64).requires_grad_(True)
+    y64 = y.to(torch.complex128)
+
+    def fwd(v):
+        return torch.stack([ref.forward((m64[c] * v)[None], k64[None])[0] for c in range(C)])
+
+    def adj(w):
+        return sum(m64[c].conj() * ref.adjoint(w[c][None], k64[None])[0] for c in range(C))
+
+    l64 = torch.sum(torch.abs(fwd(x64) - y64) ** 2) + 1e-3 * torch.sum(torch.abs(adj(fwd(x64))) ** 2) \
+        + 1e-2 * torch.sum(torch.abs(adj(y64)) ** 2)
+    l64.backward()
+    assert abs(float(loss.detach()) - float(l64)) / float(l64) < 1e-4
     assert rel_l2(xd.grad, x64.grad) < 1e-4
     assert rel_l2(kd.grad.to(torch.float64), k64.grad) < 1e-3
 
