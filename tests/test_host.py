@@ -226,3 +226,32 @@ def test_gridded_scatter_chunks_vs_groupby():
                 want.append((pos + s, pos + min(s + chunk, n), key))
             pos += n
         assert list(zip(cs.tolist(), ce.tolist(), ci.tolist())) == want
+
+
+def test_cg_step_entry_points_validate_before_any_launch():
+    """The differentiable CG step's entry points refuse empty vectors and null buffers with a
+    message, before touching a device (so this runs without a GPU)."""
+    from mr_recon_amd import _lib
+
+    L = _lib.lib()
+    rc = L.mra_cg_step_fwd(None, None, None, None, None, 0, 1.0, 0.0, None, None, None, None, None, None, None)
+    assert rc != 0 and b"n must be positive" in L.mra_last_error()
+    rc = L.mra_cg_step_fwd(None, None, None, None, None, 16, 1.0, 0.0, None, None, None, None, None, None, None)
+    assert rc != 0 and b"null pointer" in L.mra_last_error()
+    rc = L.mra_cg_step_bwd(None, None, None, None, None, None, None, None, 16, 1.0, 0.0, None, None, None, None,
+                           None, None, None)
+    assert rc != 0 and b"null pointer" in L.mra_last_error()
+
+
+def test_binning_chunk_bounds():
+    """Radix chunks stay within 128..4096 samples, and the histogram (digits x chunks) within
+    2^24 counts wherever 4096-sample chunks allow it."""
+    from mr_recon_amd.plan import MAX_DIGIT, binning_chunk
+
+    for npts in (1, 100, 3000, 131072, 5_120_000, 50_000_000):
+        for ntiles, tile, d in ((27, 8, 3), (400, 16, 2), (8000, 8, 3), (64000, 8, 3)):
+            c = binning_chunk(npts, ntiles, tile, d)
+            assert 128 <= c <= 4096
+            digits = max(tile ** d, min(ntiles + 1, MAX_DIGIT))
+            chunks = -(-npts // c)
+            assert c == 4096 or digits * chunks <= 1 << 24
